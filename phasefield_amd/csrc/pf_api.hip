@@ -1,0 +1,1020 @@
+// pf_api.hip — the C-ABI (include/phasefield_amd.h): domain handles, host<->device layout
+// conversion with the 16-bit grain-code remap, step orchestration on one HIP stream, FP64
+// reductions, checkpoints, device-side seeded setup, and the reference's three-function
+// accelerator API (gpu_solver.h:18-23) as a thin shim over the same kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/phasefield_amd.h"
+#include "pf_kernels.h"
+
+using namespace pfamd;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct pf_handle {
+  pf_desc d{};
+  long N = 0;
+  int KI = 4, NF = 6, dims = 3;
+  size_t tsz = 8, csz = 8;  // bytes of one phi/value_phase element and one stored concentration
+  hipStream_t stream = nullptr;
+  uint16_t *id[2] = {nullptr, nullptr};
+  void *phi[2] = {nullptr, nullptr};
+  void *pfrac = nullptr;
+  void *conc[PF_MAX_SOLUTE][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int cur = 0;                       // active slot buffer
+  int ccur[PF_MAX_SOLUTE] = {0, 0};  // active concentration buffer per solute
+  RedPartial *red = nullptr;
+  int red_blocks = 0;
+  pf_params p{};
+  bool have_params = false, have_state = false;
+  std::vector<int32_t> code_to_id[4];  // per phase: serial -> reference grain id
+  std::string err;
+  // timing of the dominant launches
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
+  size_t ev_used[2] = {0, 0};
+  double t_ms[2] = {0, 0};
+  int64_t t_n[2] = {0, 0};
+  int diff_zchunk_override = 0;
+};
+
+namespace {
+
+int fail(pf_handle *h, int code, const std::string &msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail((h), e_ == hipErrorOutOfMemory ? PF_ERR_OOM : PF_ERR_HIP,                  \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                        \
+  } while (0)
+
+bool is_mixed(const pf_handle *h) { return h->d.precision == PF_MIXED; }
+
+// ---- constants ----------------------------------------------------------------------------------
+template <class A> PFConsts<A> make_pf_consts(const pf_handle *h) {
+  const pf_params &p = h->p;
+  PFConsts<A> c;
+  std::memset(&c, 0, sizeof(c));
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const bool in = i > 0 && j > 0;
+      c.M[i][j] = A(in ? p.mobility[i - 1][j - 1] : 0.0);
+      c.sig[i][j] = A(in ? p.sigma[i - 1][j - 1] : 0.0);
+      c.ent[i][j] = A(in ? p.entropy[i - 1][j - 1] : 0.0);
+      c.eqc[i][j] = A(in ? p.eq_conc[i - 1][j - 1] : 0.0);
+      c.slope[i][j] = A(in ? p.eq_slope[i - 1][j - 1] : 1.0);
+      c.pk[i][j] = A(in ? p.part_k[0][i - 1][j - 1] : 0.0);
+      c.pc[i][j] = A(in ? p.part_c[0][i - 1][j - 1] : 0.0);
+      c.pkS[i][j] = A(in ? p.part_k[1][i - 1][j - 1] : 0.0);
+      c.pcS[i][j] = A(in ? p.part_c[1][i - 1][j - 1] : 0.0);
+    }
+  // solver.cpp:226-228
+  c.f1 = A(1. / (p.dx * p.dx));
+  c.f2 = A(kPi * kPi / (2. * p.eta * p.eta));
+  c.f3 = A(kPi / p.eta);
+  c.dt = p.pfm_dt;
+  c.cutoff = p.cutoff;
+  return c;
+}
+
+template <class A> DiffConsts<A> make_diff_consts(const pf_params &p, int s, double factor) {
+  DiffConsts<A> d;
+  std::memset(&d, 0, sizeof(d));
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      d.pk[i][j] = A(p.part_k[s][i][j]);
+      d.pc[i][j] = A(p.part_c[s][i][j]);
+    }
+    d.D[i] = A(p.diffusivity[s][i]);
+  }
+  d.r = A(factor);
+  return d;
+}
+
+// ---- launch helpers -----------------------------------------------------------------------------
+inline unsigned blocks_for(long n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+
+struct Timer {
+  pf_handle *h;
+  int kind;
+  hipEvent_t b = nullptr, e = nullptr;
+  Timer(pf_handle *hh, int k) : h(hh), kind(k) {
+    if (!h->timing) return;
+    auto &pool = h->ev[kind];
+    if (h->ev_used[kind] == pool.size()) {
+      hipEvent_t x, y;
+      if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) return;
+      pool.emplace_back(x, y);
+    }
+    b = pool[h->ev_used[kind]].first;
+    e = pool[h->ev_used[kind]].second;
+    h->ev_used[kind]++;
+    hipEventRecord(b, h->stream);
+  }
+  ~Timer() {
+    if (e) hipEventRecord(e, h->stream);
+  }
+};
+
+template <class F, int K, int P, int NF, bool NOF, bool ISO>
+hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
+  using T = typename F::T;
+  using C = typename F::C;
+  const int a = h->cur, b = 1 - h->cur;
+  hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, ISO>), dim3(blocks_for(h->N, kSweepBS)),
+                     dim3(kSweepBS), 0, h->stream, h->id[a], static_cast<const T *>(h->phi[a]), h->id[b],
+                     static_cast<T *>(h->phi[b]), static_cast<T *>(h->pfrac),
+                     static_cast<const C *>(h->conc[0][h->ccur[0]]), cs, h->d.nx, h->d.ny, h->d.nz,
+                     h->N);
+  h->cur = b;
+  return hipGetLastError();
+}
+
+template <class F, int K, int P, bool ISO>
+hipError_t dispatch_sweep_nf(pf_handle *h, const PFConsts<typename F::A> &cs) {
+  if (h->NF == 4) return launch_sweep<F, K, P, 4, false, ISO>(h, cs);
+  if (h->d.bc_z == PF_BC_NOFLUX) return launch_sweep<F, K, P, 6, true, ISO>(h, cs);
+  return launch_sweep<F, K, P, 6, false, ISO>(h, cs);
+}
+
+template <class F, bool ISO>
+hipError_t dispatch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
+  const int K = h->d.depth, P = h->d.nphase;
+  if (K == 4 && P == 2) return dispatch_sweep_nf<F, 4, 2, ISO>(h, cs);
+  if (K == 4 && P == 3) return dispatch_sweep_nf<F, 4, 3, ISO>(h, cs);
+  if (K == 6 && P == 2) return dispatch_sweep_nf<F, 6, 2, ISO>(h, cs);
+  return dispatch_sweep_nf<F, 6, 3, ISO>(h, cs);
+}
+
+struct DiffGeom {
+  int tiles_x, tiles_y, zchunk, nblocks;
+};
+
+DiffGeom diff_geom(const pf_handle *h) {
+  DiffGeom g;
+  g.tiles_x = (h->d.nx + 63) / 64;
+  g.tiles_y = (h->d.ny + kDiffTY - 1) / kDiffTY;
+  const int nz = h->d.nz;
+  int zc;
+  if (h->NF != 6) {
+    zc = nz;  // planes are independent; one chunk walks all of them
+  } else if (h->diff_zchunk_override > 0) {
+    zc = std::min(h->diff_zchunk_override, nz);
+  } else {
+    const long tiles = long(g.tiles_x) * g.tiles_y;
+    long nzc = std::max(1L, std::min<long>(nz, (2048 + tiles - 1) / tiles));
+    zc = int((nz + nzc - 1) / nzc);
+    zc = std::max(zc, std::min(nz, 16));
+  }
+  g.zchunk = zc;
+  g.nblocks = g.tiles_x * g.tiles_y * ((nz + zc - 1) / zc);
+  return g;
+}
+
+template <class F, int P, int NF, bool NOF>
+hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &dc) {
+  using T = typename F::T;
+  using C = typename F::C;
+  const DiffGeom g = diff_geom(h);
+  const int a = h->ccur[s], b = 1 - a;
+  hipLaunchKernelGGL((diffuse_kernel<F, P, NF, NOF>), dim3(g.nblocks), dim3(64, kDiffTY), 0,
+                     h->stream, static_cast<const C *>(h->conc[s][a]),
+                     static_cast<C *>(h->conc[s][b]), static_cast<const T *>(h->pfrac), dc,
+                     h->d.nx, h->d.ny, h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+  h->ccur[s] = b;
+  return hipGetLastError();
+}
+
+template <class F, int P>
+hipError_t dispatch_diffuse_p(pf_handle *h, int s, const DiffConsts<typename F::A> &dc) {
+  if (h->NF == 4) return launch_diffuse<F, P, 4, false>(h, s, dc);
+  if (h->d.bc_z == PF_BC_NOFLUX) return launch_diffuse<F, P, 6, true>(h, s, dc);
+  return launch_diffuse<F, P, 6, false>(h, s, dc);
+}
+
+template <class F>
+int run_diffuse(pf_handle *h, int s, int n, double factor) {
+  const auto dc = make_diff_consts<typename F::A>(h->p, s, factor);
+  for (int it = 0; it < n; ++it) {
+    Timer t(h, 0);
+    hipError_t e = h->d.nphase == 2 ? dispatch_diffuse_p<F, 2>(h, s, dc)
+                                    : dispatch_diffuse_p<F, 3>(h, s, dc);
+    if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("diffuse: ") + hipGetErrorString(e));
+  }
+  return PF_OK;
+}
+
+template <class F> int run_sweep(pf_handle *h) {
+  const auto cs = make_pf_consts<typename F::A>(h);
+  Timer t(h, 1);
+  hipError_t e = dispatch_sweep<F, false>(h, cs);
+  if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("sweep: ") + hipGetErrorString(e));
+  return PF_OK;
+}
+
+int check_ready(pf_handle *h) {
+  if (!h) return PF_ERR_INVALID;
+  if (!h->have_params) return fail(h, PF_ERR_STATE, "parameters not set (pf_set_params)");
+  if (!h->have_state) return fail(h, PF_ERR_STATE, "state not uploaded or constructed");
+  return PF_OK;
+}
+
+int diffuse_any(pf_handle *h, int s, int n, double factor) {
+  return is_mixed(h) ? run_diffuse<FormMixed>(h, s, n, factor)
+                     : run_diffuse<FormF64>(h, s, n, factor);
+}
+
+void free_all(pf_handle *h) {
+  for (int b = 0; b < 2; ++b) {
+    if (h->id[b]) hipFree(h->id[b]);
+    if (h->phi[b]) hipFree(h->phi[b]);
+    for (int s = 0; s < PF_MAX_SOLUTE; ++s)
+      if (h->conc[s][b]) hipFree(h->conc[s][b]);
+  }
+  if (h->pfrac) hipFree(h->pfrac);
+  if (h->red) hipFree(h->red);
+  for (int k = 0; k < 2; ++k)
+    for (auto &pr : h->ev[k]) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+  if (h->stream) hipStreamDestroy(h->stream);
+}
+
+int phase_of_id(int32_t g) {  // phase_field.h:25-29
+  if (g == 0) return 0;
+  return g / 100000 > 0 ? g / 100000 : 1;
+}
+
+}  // namespace
+
+// =================================================================================================
+extern "C" {
+
+const char *pf_last_error(const pf_handle *h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int pf_create(const pf_desc *desc, pf_handle **out) {
+  if (!desc || !out) return fail(nullptr, PF_ERR_INVALID, "null argument");
+  const pf_desc &d = *desc;
+  if (d.nx < 1 || d.ny < 1 || d.nz < 1)
+    return fail(nullptr, PF_ERR_INVALID, "domain extents must be >= 1");
+  if (d.depth != 4 && d.depth != 6) return fail(nullptr, PF_ERR_INVALID, "depth must be 4 or 6");
+  if (d.nphase != 2 && d.nphase != 3) return fail(nullptr, PF_ERR_INVALID, "nphase must be 2 or 3");
+  if (d.nsolute < 1 || d.nsolute > PF_MAX_SOLUTE)
+    return fail(nullptr, PF_ERR_INVALID, "nsolute must be 1 or 2");
+  if (d.precision != PF_FP64 && d.precision != PF_MIXED)
+    return fail(nullptr, PF_ERR_INVALID, "precision must be PF_FP64 or PF_MIXED");
+  if (d.bc_z != PF_BC_PERIODIC && d.bc_z != PF_BC_NOFLUX)
+    return fail(nullptr, PF_ERR_INVALID, "bc_z must be periodic or no-flux");
+  const int dims = 3 - (d.nz == 1 ? 1 : 0) - (d.ny == 1 ? 1 : 0);
+  if (dims < 2) return fail(nullptr, PF_ERR_INVALID, "one-dimensional domains are not supported");
+  if (d.bc_z == PF_BC_NOFLUX && dims != 3)
+    return fail(nullptr, PF_ERR_INVALID, "no-flux z needs a 3D domain");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, PF_ERR_NODEVICE, "no HIP device available");
+  if (d.device < 0 || d.device >= ndev) return fail(nullptr, PF_ERR_INVALID, "bad device ordinal");
+  if (hipSetDevice(d.device) != hipSuccess) return fail(nullptr, PF_ERR_HIP, "hipSetDevice failed");
+
+  auto h = std::make_unique<pf_handle>();
+  h->d = d;
+  h->N = long(d.nx) * d.ny * d.nz;
+  h->KI = d.depth == 4 ? 4 : 8;
+  h->dims = dims;
+  h->NF = 2 * dims;
+  h->tsz = d.precision == PF_MIXED ? 4 : 8;
+  h->csz = 8;  // double, or float2 {value, compensation}
+  if (const char *e = std::getenv("PFAMD_DIFF_ZCHUNK")) h->diff_zchunk_override = std::atoi(e);
+  pf_handle *hp = h.get();
+  auto chk = [&](hipError_t e, const char *what) -> int {
+    if (e == hipSuccess) return PF_OK;
+    free_all(hp);
+    return fail(nullptr, e == hipErrorOutOfMemory ? PF_ERR_OOM : PF_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+  };
+  int rc;
+  if ((rc = chk(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) != PF_OK)
+    return rc;
+  const size_t N = size_t(h->N);
+  for (int b = 0; b < 2; ++b) {
+    if ((rc = chk(hipMalloc(&h->id[b], N * h->KI * 2), "alloc ids")) != PF_OK) return rc;
+    if ((rc = chk(hipMalloc(&h->phi[b], N * d.depth * h->tsz), "alloc phi")) != PF_OK) return rc;
+    for (int s = 0; s < d.nsolute; ++s)
+      if ((rc = chk(hipMalloc(&h->conc[s][b], N * h->csz), "alloc conc")) != PF_OK) return rc;
+  }
+  if ((rc = chk(hipMalloc(&h->pfrac, N * d.nphase * h->tsz), "alloc value_phase")) != PF_OK)
+    return rc;
+  h->red_blocks = int(std::min<long>(1024, std::max<long>(1, (h->N + 4095) / 4096)));
+  if ((rc = chk(hipMalloc(&h->red, sizeof(RedPartial) * h->red_blocks), "alloc partials")) != PF_OK)
+    return rc;
+  for (int b = 0; b < 2; ++b) {
+    if ((rc = chk(hipMemsetAsync(h->id[b], 0, N * h->KI * 2, h->stream), "clear")) != PF_OK)
+      return rc;
+    if ((rc = chk(hipMemsetAsync(h->phi[b], 0, N * d.depth * h->tsz, h->stream), "clear")) != PF_OK)
+      return rc;
+  }
+  if ((rc = chk(hipStreamSynchronize(h->stream), "sync")) != PF_OK) return rc;
+  *out = h.release();
+  return PF_OK;
+}
+
+int pf_destroy(pf_handle *h) {
+  if (!h) return PF_OK;
+  hipSetDevice(h->d.device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return PF_OK;
+}
+
+int pf_set_params(pf_handle *h, const pf_params *p) {
+  if (!h || !p) return PF_ERR_INVALID;
+  if (!(p->dx > 0) || !(p->eta > 0) || !(p->pfm_dt > 0))
+    return fail(h, PF_ERR_INVALID, "dx, eta and pfm_dt must be positive");
+  if (p->dims != h->dims)
+    return fail(h, PF_ERR_INVALID, "params.dims does not match the domain shape");
+  for (int s = 0; s < h->d.nsolute; ++s)
+    if (p->n_sub[s] < 0) return fail(h, PF_ERR_INVALID, "negative sub-step count");
+  h->p = *p;
+  h->have_params = true;
+  return PF_OK;
+}
+
+int pf_upload_state(pf_handle *h, const pf_host_state *s) {
+  if (!h || !s || !s->index || !s->value || !s->phase || !s->conc)
+    return fail(h, PF_ERR_INVALID, "null state pointer");
+  hipSetDevice(h->d.device);
+  const long N = h->N;
+  const int K = h->d.depth, KI = h->KI, P = h->d.nphase;
+  // grain-code tables: per phase, sorted unique ids -> serials 1..
+  std::vector<int32_t> ids[4];
+  for (long n = 0; n < N * K; ++n) {
+    const int32_t g = s->index[n];
+    if (g == 0) continue;
+    if (g < 0) return fail(h, PF_ERR_INVALID, "negative grain id");
+    const int ph = phase_of_id(g);
+    if (ph > 3 || ph > P)
+      return fail(h, PF_ERR_INVALID, "grain id of a phase the domain does not carry");
+    ids[ph].push_back(g);
+  }
+  std::map<int32_t, uint16_t> code;
+  for (int ph = 1; ph <= 3; ++ph) {
+    std::sort(ids[ph].begin(), ids[ph].end());
+    ids[ph].erase(std::unique(ids[ph].begin(), ids[ph].end()), ids[ph].end());
+    if (ids[ph].size() > 16383)
+      return fail(h, PF_ERR_RANGE, "more than 16383 grains of one phase");
+    h->code_to_id[ph].assign(1, 0);
+    for (size_t q = 0; q < ids[ph].size(); ++q) {
+      h->code_to_id[ph].push_back(ids[ph][q]);
+      code[ids[ph][q]] = uint16_t((ph << 14) | int(q + 1));
+    }
+  }
+  std::vector<uint16_t> hid(size_t(N) * KI, 0);
+  for (long n = 0; n < N; ++n)
+    for (int k = 0; k < K; ++k) {
+      const int32_t g = s->index[n * K + k];
+      hid[size_t(n) * KI + k] = g == 0 ? 0 : code[g];
+    }
+  for (long n = 0; n < N; ++n)
+    if (P == 2 && s->phase[n * 3 + 2] != 0.0)
+      return fail(h, PF_ERR_INVALID, "value_phase of phase 3 is nonzero but nphase == 2");
+  const size_t NK = size_t(N) * K;
+  HIPCHK(h, hipMemcpyAsync(h->id[0], hid.data(), hid.size() * 2, hipMemcpyHostToDevice, h->stream));
+  if (is_mixed(h)) {
+    std::vector<float> v(NK), ph(size_t(N) * P);
+    for (size_t n = 0; n < NK; ++n) v[n] = float(s->value[n]);
+    for (long n = 0; n < N; ++n)
+      for (int p = 0; p < P; ++p) ph[size_t(n) * P + p] = float(s->phase[n * 3 + p]);
+    HIPCHK(h, hipMemcpyAsync(h->phi[0], v.data(), NK * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->pfrac, ph.data(), ph.size() * 4, hipMemcpyHostToDevice, h->stream));
+    for (int sol = 0; sol < h->d.nsolute; ++sol) {
+      std::vector<float2> c(N);
+      for (long n = 0; n < N; ++n)
+        c[n] = make_float2(float(s->conc[sol * N + n]),
+                           s->conc_comp ? float(s->conc_comp[sol * N + n]) : 0.f);
+      HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), N * 8, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  } else {
+    std::vector<double> ph(size_t(N) * P);
+    for (long n = 0; n < N; ++n)
+      for (int p = 0; p < P; ++p) ph[size_t(n) * P + p] = s->phase[n * 3 + p];
+    HIPCHK(h, hipMemcpyAsync(h->phi[0], s->value, NK * 8, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->pfrac, ph.data(), ph.size() * 8, hipMemcpyHostToDevice, h->stream));
+    for (int sol = 0; sol < h->d.nsolute; ++sol)
+      HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], s->conc + sol * N, N * 8, hipMemcpyHostToDevice,
+                               h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->cur = 0;
+  h->ccur[0] = h->ccur[1] = 0;
+  h->have_state = true;
+  return PF_OK;
+}
+
+int pf_download_state(pf_handle *h, pf_host_state *s) {
+  if (!h || !s) return PF_ERR_INVALID;
+  if (!h->have_state) return fail(h, PF_ERR_STATE, "no state on the device");
+  hipSetDevice(h->d.device);
+  const long N = h->N;
+  const int K = h->d.depth, KI = h->KI, P = h->d.nphase;
+  const size_t NK = size_t(N) * K;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (s->index) {
+    std::vector<uint16_t> hid(size_t(N) * KI);
+    HIPCHK(h, hipMemcpy(hid.data(), h->id[h->cur], hid.size() * 2, hipMemcpyDeviceToHost));
+    for (long n = 0; n < N; ++n)
+      for (int k = 0; k < K; ++k) {
+        const uint16_t c = hid[size_t(n) * KI + k];
+        const int ph = c >> 14, ser = c & 0x3fff;
+        int32_t g = 0;
+        if (c != 0) {
+          if (ser >= int(h->code_to_id[ph].size()))
+            return fail(h, PF_ERR_STATE, "grain code without a table entry");
+          g = h->code_to_id[ph][ser];
+        }
+        s->index[n * K + k] = g;
+      }
+  }
+  std::vector<double> phase(size_t(N) * 3, 0.0);
+  std::vector<double> conc(size_t(N) * h->d.nsolute), comp(size_t(N) * h->d.nsolute, 0.0);
+  if (is_mixed(h)) {
+    std::vector<float> v(NK), ph(size_t(N) * P);
+    HIPCHK(h, hipMemcpy(v.data(), h->phi[h->cur], NK * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ph.data(), h->pfrac, ph.size() * 4, hipMemcpyDeviceToHost));
+    if (s->value)
+      for (size_t n = 0; n < NK; ++n) s->value[n] = v[n];
+    for (long n = 0; n < N; ++n)
+      for (int p = 0; p < P; ++p) phase[size_t(n) * 3 + p] = ph[size_t(n) * P + p];
+    for (int sol = 0; sol < h->d.nsolute; ++sol) {
+      std::vector<float2> c(N);
+      HIPCHK(h, hipMemcpy(c.data(), h->conc[sol][h->ccur[sol]], N * 8, hipMemcpyDeviceToHost));
+      for (long n = 0; n < N; ++n) {
+        conc[sol * N + n] = c[n].x;
+        comp[sol * N + n] = c[n].y;
+      }
+    }
+  } else {
+    std::vector<double> ph(size_t(N) * P);
+    if (s->value)
+      HIPCHK(h, hipMemcpy(s->value, h->phi[h->cur], NK * 8, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ph.data(), h->pfrac, ph.size() * 8, hipMemcpyDeviceToHost));
+    for (long n = 0; n < N; ++n)
+      for (int p = 0; p < P; ++p) phase[size_t(n) * 3 + p] = ph[size_t(n) * P + p];
+    for (int sol = 0; sol < h->d.nsolute; ++sol)
+      HIPCHK(h, hipMemcpy(conc.data() + sol * N, h->conc[sol][h->ccur[sol]], N * 8,
+                          hipMemcpyDeviceToHost));
+  }
+  if (s->phase) std::memcpy(s->phase, phase.data(), phase.size() * 8);
+  if (s->conc) std::memcpy(s->conc, conc.data(), conc.size() * 8);
+  if (s->conc_comp) std::memcpy(s->conc_comp, comp.data(), comp.size() * 8);
+  if (s->phase_conc) {
+    // value_conc1 = Partitioning(total_conc, value_phase), solver.cpp:389-405, in the form's
+    // precision over all three phases
+    for (int sol = 0; sol < h->d.nsolute; ++sol)
+      for (long n = 0; n < N; ++n)
+        for (int i = 0; i < 3; ++i) {
+          if (is_mixed(h)) {
+            float t1 = 0.f, t2 = 0.f;
+            for (int j = 0; j < 3; ++j) {
+              t1 += float(phase[n * 3 + j]) * float(h->p.part_k[sol][i][j]);
+              t2 += float(phase[n * 3 + j]) * float(h->p.part_c[sol][i][j]);
+            }
+            s->phase_conc[(sol * N + n) * 3 + i] = (float(conc[sol * N + n]) - t2) / t1;
+          } else {
+            double t1 = 0.0, t2 = 0.0;
+            for (int j = 0; j < 3; ++j) {
+              t1 += phase[n * 3 + j] * h->p.part_k[sol][i][j];
+              t2 += phase[n * 3 + j] * h->p.part_c[sol][i][j];
+            }
+            s->phase_conc[(sol * N + n) * 3 + i] = (conc[sol * N + n] - t2) / t1;
+          }
+        }
+  }
+  return PF_OK;
+}
+
+int pf_phase_field_sweep(pf_handle *h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  hipSetDevice(h->d.device);
+  return is_mixed(h) ? run_sweep<FormMixed>(h) : run_sweep<FormF64>(h);
+}
+
+int pf_diffuse_substeps(pf_handle *h, int solute, int n, double dt_d) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (solute < 0 || solute >= h->d.nsolute || n < 0)
+    return fail(h, PF_ERR_INVALID, "bad solute index or sub-step count");
+  hipSetDevice(h->d.device);
+  // phase_field.cpp:366: del_conc *= diff_dt/(dx*dx)
+  return diffuse_any(h, solute, n, dt_d / (h->p.dx * h->p.dx));
+}
+
+int pf_step(pf_handle *h, int n_steps) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  hipSetDevice(h->d.device);
+  for (int it = 0; it < n_steps; ++it) {
+    rc = is_mixed(h) ? run_sweep<FormMixed>(h) : run_sweep<FormF64>(h);
+    if (rc) return rc;
+    for (int s = 0; s < h->d.nsolute; ++s) {
+      if (h->p.n_sub[s] <= 0) continue;
+      rc = diffuse_any(h, s, h->p.n_sub[s], h->p.dt_d[s] / (h->p.dx * h->p.dx));
+      if (rc) return rc;
+    }
+  }
+  return PF_OK;
+}
+
+int pf_synchronize(pf_handle *h) {
+  if (!h) return PF_ERR_INVALID;
+  hipSetDevice(h->d.device);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PF_OK;
+}
+
+int pf_reduce(pf_handle *h, pf_stats *out) {
+  if (!h || !out) return PF_ERR_INVALID;
+  if (!h->have_state) return fail(h, PF_ERR_STATE, "no state on the device");
+  hipSetDevice(h->d.device);
+  const long per = (h->N + h->red_blocks - 1) / h->red_blocks;
+  const int nb = int((h->N + per - 1) / per);
+  const int K = h->d.depth, P = h->d.nphase;
+  const void *c1 = h->d.nsolute > 1 ? h->conc[1][h->ccur[1]] : h->conc[0][h->ccur[0]];
+#define PF_RED(F, KK, PP)                                                                          \
+  hipLaunchKernelGGL((reduce_stage1<F, KK, PP>), dim3(nb), dim3(256), 0, h->stream, h->id[h->cur], \
+                     static_cast<const F::T *>(h->phi[h->cur]),                                    \
+                     static_cast<const F::T *>(h->pfrac),                                          \
+                     static_cast<const F::C *>(h->conc[0][h->ccur[0]]),                            \
+                     static_cast<const F::C *>(c1), h->d.nsolute, h->N, per, h->red)
+  if (is_mixed(h)) {
+    if (K == 4 && P == 2) PF_RED(FormMixed, 4, 2);
+    else if (K == 4) PF_RED(FormMixed, 4, 3);
+    else if (P == 2) PF_RED(FormMixed, 6, 2);
+    else PF_RED(FormMixed, 6, 3);
+  } else {
+    if (K == 4 && P == 2) PF_RED(FormF64, 4, 2);
+    else if (K == 4) PF_RED(FormF64, 4, 3);
+    else if (P == 2) PF_RED(FormF64, 6, 2);
+    else PF_RED(FormF64, 6, 3);
+  }
+#undef PF_RED
+  HIPCHK(h, hipGetLastError());
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(256), 0, h->stream, h->red, nb);
+  HIPCHK(h, hipGetLastError());
+  RedPartial r;
+  HIPCHK(h, hipMemcpyAsync(&r, h->red, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memset(out, 0, sizeof(*out));
+  for (int s = 0; s < PF_MAX_SOLUTE; ++s) {
+    const bool in = s < h->d.nsolute;
+    out->mass[s] = in ? r.mass[s] : 0.0;
+    out->max_conc[s] = in ? r.cmax[s] : 0.0;
+    out->min_conc[s] = in ? r.cmin[s] : 0.0;
+  }
+  for (int p = 0; p < 3; ++p) out->phase_frac[p] = r.pf[p] / double(h->N);
+  out->max_sum_err = r.serr;
+  out->n_interface = r.nif;
+  out->n_nonfinite = r.nnf;
+  if (r.nnf > 0) return fail(h, PF_ERR_NONFINITE, "non-finite values in the state");
+  return PF_OK;
+}
+
+int pf_rex_fraction(pf_handle *h, double *out) {
+  if (!h || !out) return PF_ERR_INVALID;
+  bool any = false;
+  for (int ph = 0; ph < 4; ++ph)
+    for (size_t q = 1; q < h->code_to_id[ph].size(); ++q) any = any || h->code_to_id[ph][q] < 100000;
+  *out = 0.0;
+  if (!any) return PF_OK;
+  const long N = h->N;
+  const int K = h->d.depth;
+  std::vector<int32_t> idx(size_t(N) * K);
+  std::vector<double> val(size_t(N) * K);
+  std::vector<double> phase(size_t(N) * 3), conc(size_t(N) * h->d.nsolute);
+  pf_host_state s{};
+  s.index = idx.data();
+  s.value = val.data();
+  s.phase = phase.data();
+  s.conc = conc.data();
+  int rc = pf_download_state(h, &s);
+  if (rc != PF_OK) return rc;
+  double acc = 0.0;
+  for (size_t n = 0; n < idx.size(); ++n) acc += idx[n] < 100000 ? val[n] : 0.0;
+  *out = acc / double(N);
+  return PF_OK;
+}
+
+int pf_kernel_timing_reset(pf_handle *h, int enable) {
+  if (!h) return PF_ERR_INVALID;
+  hipSetDevice(h->d.device);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->timing = enable != 0;
+  h->ev_used[0] = h->ev_used[1] = 0;
+  h->t_ms[0] = h->t_ms[1] = 0.0;
+  h->t_n[0] = h->t_n[1] = 0;
+  return PF_OK;
+}
+
+int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches) {
+  if (!h || kind < 0 || kind > 1) return PF_ERR_INVALID;
+  hipSetDevice(h->d.device);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < h->ev_used[kind]; ++i) {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev[kind][i].first, h->ev[kind][i].second));
+    h->t_ms[kind] += ms;
+    h->t_n[kind] += 1;
+  }
+  h->ev_used[kind] = 0;
+  if (total_ms) *total_ms = h->t_ms[kind];
+  if (launches) *launches = h->t_n[kind];
+  return PF_OK;
+}
+
+// ---- checkpoints (host memory, raw device layout) -----------------------------------------------
+int pf_checkpoint_size(const pf_handle *h, size_t *bytes) {
+  if (!h || !bytes) return PF_ERR_INVALID;
+  const size_t N = size_t(h->N);
+  *bytes = 64 + N * h->KI * 2 + N * h->d.depth * h->tsz + N * h->d.nphase * h->tsz +
+           N * h->csz * h->d.nsolute;
+  return PF_OK;
+}
+
+int pf_checkpoint_save(pf_handle *h, void *dst) {
+  if (!h || !dst) return PF_ERR_INVALID;
+  if (!h->have_state) return fail(h, PF_ERR_STATE, "no state on the device");
+  hipSetDevice(h->d.device);
+  const size_t N = size_t(h->N);
+  char *p = static_cast<char *>(dst);
+  std::memset(p, 0, 64);
+  std::memcpy(p, "PFCKPT01", 8);
+  std::memcpy(p + 8, &h->d, sizeof(pf_desc));
+  p += 64;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(p, h->id[h->cur], N * h->KI * 2, hipMemcpyDeviceToHost));
+  p += N * h->KI * 2;
+  HIPCHK(h, hipMemcpy(p, h->phi[h->cur], N * h->d.depth * h->tsz, hipMemcpyDeviceToHost));
+  p += N * h->d.depth * h->tsz;
+  HIPCHK(h, hipMemcpy(p, h->pfrac, N * h->d.nphase * h->tsz, hipMemcpyDeviceToHost));
+  p += N * h->d.nphase * h->tsz;
+  for (int s = 0; s < h->d.nsolute; ++s) {
+    HIPCHK(h, hipMemcpy(p, h->conc[s][h->ccur[s]], N * h->csz, hipMemcpyDeviceToHost));
+    p += N * h->csz;
+  }
+  return PF_OK;
+}
+
+int pf_checkpoint_restore(pf_handle *h, const void *src) {
+  if (!h || !src) return PF_ERR_INVALID;
+  hipSetDevice(h->d.device);
+  const size_t N = size_t(h->N);
+  const char *p = static_cast<const char *>(src);
+  pf_desc d;
+  std::memcpy(&d, p + 8, sizeof(d));
+  if (std::memcmp(p, "PFCKPT01", 8) != 0 || std::memcmp(&d, &h->d, sizeof(d)) != 0)
+    return fail(h, PF_ERR_INVALID, "checkpoint does not match this domain");
+  p += 64;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy(h->id[h->cur], p, N * h->KI * 2, hipMemcpyHostToDevice));
+  p += N * h->KI * 2;
+  HIPCHK(h, hipMemcpy(h->phi[h->cur], p, N * h->d.depth * h->tsz, hipMemcpyHostToDevice));
+  p += N * h->d.depth * h->tsz;
+  HIPCHK(h, hipMemcpy(h->pfrac, p, N * h->d.nphase * h->tsz, hipMemcpyHostToDevice));
+  p += N * h->d.nphase * h->tsz;
+  for (int s = 0; s < h->d.nsolute; ++s) {
+    HIPCHK(h, hipMemcpy(h->conc[s][h->ccur[s]], p, N * h->csz, hipMemcpyHostToDevice));
+    p += N * h->csz;
+  }
+  h->have_state = true;
+  return PF_OK;
+}
+
+// ---- device-side seeded setup -----------------------------------------------------------------
+int pf_setup_construct(pf_handle *h, const double *seeds, int G, double reduction, int n_iso,
+                       const double *init_conc) {
+  if (!h || !seeds || G < 1 || !init_conc) return fail(h, PF_ERR_INVALID, "bad setup arguments");
+  if (is_mixed(h))
+    return fail(h, PF_ERR_INVALID,
+                "set up the microstructure on an FP64 domain and convert it with pf_copy_state");
+  if (!h->have_params) return fail(h, PF_ERR_STATE, "parameters not set (pf_set_params)");
+  if (G > 16383) return fail(h, PF_ERR_RANGE, "more than 16383 Voronoi grains");
+  if (reduction >= 1.0) return fail(h, PF_ERR_INVALID, "reduction must be < 1");
+  hipSetDevice(h->d.device);
+  const int K = h->d.depth, P = h->d.nphase;
+  double *dseeds = nullptr;
+  HIPCHK(h, hipMalloc(&dseeds, sizeof(double) * 3 * G));
+  hipError_t e = hipMemcpyAsync(dseeds, seeds, sizeof(double) * 3 * G, hipMemcpyHostToDevice,
+                                h->stream);
+  double *phi = static_cast<double *>(h->phi[0]), *pf = static_cast<double *>(h->pfrac);
+  if (e == hipSuccess) {
+    h->cur = 0;
+#define PF_VOR(KK, PP)                                                                        \
+  hipLaunchKernelGGL((voronoi_kernel<KK, PP>), dim3(blocks_for(h->N)), dim3(256), 0, h->stream, \
+                     dseeds, G, reduction, h->d.nx, h->d.ny, h->d.nz, h->N, h->id[0], phi, pf)
+    if (K == 4 && P == 2) PF_VOR(4, 2);
+    else if (K == 4) PF_VOR(4, 3);
+    else if (P == 2) PF_VOR(6, 2);
+    else PF_VOR(6, 3);
+#undef PF_VOR
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(dseeds);
+  if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("voronoi: ") + hipGetErrorString(e));
+  // slot-1 buffer must hold zeros where Iso_growth leaves slots unwritten
+  HIPCHK(h, hipMemsetAsync(h->id[1], 0, size_t(h->N) * h->KI * 2, h->stream));
+  HIPCHK(h, hipMemsetAsync(h->phi[1], 0, size_t(h->N) * K * 8, h->stream));
+  // Iso_growth iterations with factor 0.5/dims*dx*dx (MicrostructureConstruction.cpp:216)
+  PFConsts<double> cs = make_pf_consts<double>(h);
+  cs.dt = 0.5 / double(h->dims) * h->p.dx * h->p.dx;
+  for (int it = 0; it < n_iso; ++it) {
+    hipError_t le = dispatch_sweep<FormF64, true>(h, cs);
+    if (le != hipSuccess) return fail(h, PF_ERR_HIP, std::string("iso: ") + hipGetErrorString(le));
+  }
+  for (int s = 0; s < h->d.nsolute; ++s) {
+    h->ccur[s] = 0;
+    double *c = static_cast<double *>(h->conc[s][0]);
+    if (P == 2)
+      hipLaunchKernelGGL((construct_conc_kernel<2>), dim3(blocks_for(h->N)), dim3(256), 0,
+                         h->stream, pf, init_conc[3 * s], init_conc[3 * s + 1],
+                         init_conc[3 * s + 2], h->N, c);
+    else
+      hipLaunchKernelGGL((construct_conc_kernel<3>), dim3(blocks_for(h->N)), dim3(256), 0,
+                         h->stream, pf, init_conc[3 * s],
+                         init_conc[3 * s + 1], init_conc[3 * s + 2], h->N, c);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int ph = 0; ph < 4; ++ph) h->code_to_id[ph].assign(1, 0);
+  for (int g = 1; g <= G; ++g) h->code_to_id[2].push_back(200000 + g);
+  h->have_state = true;
+  return PF_OK;
+}
+
+int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nucleus_conc,
+                      uint64_t seed) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (is_mixed(h)) return fail(h, PF_ERR_INVALID, "nucleate on an FP64 domain");
+  if (n_nuclei < 0 || n_nuclei > 16383) return fail(h, PF_ERR_RANGE, "bad nucleus count");
+  if (!(radius > 0.f)) return fail(h, PF_ERR_INVALID, "radius must be positive");
+  hipSetDevice(h->d.device);
+  const int K = h->d.depth, P = h->d.nphase;
+  const long N = h->N;
+  float *flag = nullptr, *pv = nullptr;
+  uint8_t *mark = nullptr;
+  long long *pi = nullptr;
+  const int nparts = int(std::min<long>(1024, (N + 4095) / 4096));
+  const long per = (N + nparts - 1) / nparts;
+  const int nb = int((N + per - 1) / per);
+  auto cleanup = [&]() {
+    if (flag) hipFree(flag);
+    if (mark) hipFree(mark);
+    if (pv) hipFree(pv);
+    if (pi) hipFree(pi);
+  };
+  hipError_t e = hipMalloc(&flag, N * 4);
+  if (e == hipSuccess) e = hipMalloc(&mark, N);
+  if (e == hipSuccess) e = hipMalloc(&pv, nparts * 4);
+  if (e == hipSuccess) e = hipMalloc(&pi, nparts * 8);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(h, PF_ERR_OOM, "nucleation scratch allocation failed");
+  }
+  const DiffConsts<double> dc = make_diff_consts<double>(h->p, 0, 0.0);
+  const uint64_t key = [&] {
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }();
+  double *phi = static_cast<double *>(h->phi[h->cur]), *pf = static_cast<double *>(h->pfrac);
+  const double *c0 = static_cast<const double *>(h->conc[0][h->ccur[0]]);
+#define PF_FLAG(KK, PP)                                                                     \
+  hipLaunchKernelGGL((nucleation_flag_kernel<KK, PP>), dim3(blocks_for(N)), dim3(256), 0,      \
+                     h->stream, h->id[h->cur], pf, c0, dc, key, N, flag, mark)
+  if (K == 4 && P == 2) PF_FLAG(4, 2);
+  else if (K == 4) PF_FLAG(4, 3);
+  else if (P == 2) PF_FLAG(6, 2);
+  else PF_FLAG(6, 3);
+#undef PF_FLAG
+  e = hipGetLastError();
+  const int R = int(std::ceil(2.0f * radius));
+  const int ext[3] = {h->d.nx, h->d.ny, h->d.nz};
+  int cntv[3];
+  for (int d = 0; d < 3; ++d) cntv[d] = (2 * R + 1 >= ext[d]) ? ext[d] : 2 * R + 1;
+  const long box = long(cntv[0]) * cntv[1] * cntv[2];
+  for (int q = 1; q <= n_nuclei && e == hipSuccess; ++q) {
+    hipLaunchKernelGGL(argmax_stage1, dim3(nb), dim3(256), 0, h->stream, flag, N, per, pv, pi);
+    hipLaunchKernelGGL(argmax_stage2, dim3(1), dim3(256), 0, h->stream, pv, pi, nb);
+#define PF_PAINT(KK, PP)                                                                      \
+  hipLaunchKernelGGL((nucleus_paint_kernel<KK, PP>), dim3(blocks_for(box)), dim3(256), 0,      \
+                     h->stream, pi, q, radius, R, h->d.nx, h->d.ny, h->d.nz, cntv[0], cntv[1], \
+                     cntv[2], h->id[h->cur], phi, pf, flag, mark)
+    if (K == 4 && P == 2) PF_PAINT(4, 2);
+    else if (K == 4) PF_PAINT(4, 3);
+    else if (P == 2) PF_PAINT(6, 2);
+    else PF_PAINT(6, 3);
+#undef PF_PAINT
+    e = hipGetLastError();
+  }
+  for (int s = 0; s < h->d.nsolute && e == hipSuccess; ++s)
+    if (nucleus_conc && nucleus_conc[s] >= 0.0) {
+      hipLaunchKernelGGL(nucleus_conc_kernel, dim3(blocks_for(N)), dim3(256), 0, h->stream, mark,
+                         nucleus_conc[s], N, static_cast<double *>(h->conc[s][h->ccur[s]]));
+      e = hipGetLastError();
+    }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("nucleate: ") + hipGetErrorString(e));
+  h->code_to_id[1].assign(1, 0);
+  for (int q = 1; q <= n_nuclei; ++q) h->code_to_id[1].push_back(100000 + q);
+  return PF_OK;
+}
+
+int pf_setup_solute_noise(pf_handle *h, int solute, double mean, double stddev, uint64_t seed) {
+  if (!h) return PF_ERR_INVALID;
+  if (!h->have_state) return fail(h, PF_ERR_STATE, "no state on the device");
+  if (is_mixed(h)) return fail(h, PF_ERR_INVALID, "set solute noise on an FP64 domain");
+  if (solute < 0 || solute >= h->d.nsolute) return fail(h, PF_ERR_INVALID, "bad solute index");
+  hipSetDevice(h->d.device);
+  uint64_t z = seed + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  const uint64_t key = z ^ (z >> 31);
+  hipLaunchKernelGGL(solute_noise_kernel, dim3(blocks_for(h->N)), dim3(256), 0, h->stream, mean,
+                     stddev, key, h->N, static_cast<double *>(h->conc[solute][h->ccur[solute]]));
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PF_OK;
+}
+
+int pf_copy_state(pf_handle *dst, const pf_handle *src) {
+  if (!dst || !src) return PF_ERR_INVALID;
+  if (!src->have_state) return fail(dst, PF_ERR_STATE, "source has no state");
+  const pf_desc &a = dst->d, &b = src->d;
+  if (a.nx != b.nx || a.ny != b.ny || a.nz != b.nz || a.depth != b.depth || a.nphase != b.nphase ||
+      a.nsolute != b.nsolute || a.device != b.device)
+    return fail(dst, PF_ERR_INVALID, "domains differ in shape, depth, phases, solutes or device");
+  if (is_mixed(src)) return fail(dst, PF_ERR_INVALID, "source must be an FP64 domain");
+  hipSetDevice(a.device);
+  HIPCHK(dst, hipStreamSynchronize(src->stream));
+  const size_t N = size_t(dst->N);
+  dst->cur = 0;
+  HIPCHK(dst, hipMemcpyAsync(dst->id[0], src->id[src->cur], N * dst->KI * 2,
+                             hipMemcpyDeviceToDevice, dst->stream));
+  if (is_mixed(dst)) {
+    hipLaunchKernelGGL(convert_f64_to_f32, dim3(blocks_for(long(N) * a.depth)), dim3(256), 0,
+                       dst->stream, static_cast<const double *>(src->phi[src->cur]),
+                       static_cast<float *>(dst->phi[0]), long(N) * a.depth);
+    hipLaunchKernelGGL(convert_f64_to_f32, dim3(blocks_for(long(N) * a.nphase)), dim3(256), 0,
+                       dst->stream, static_cast<const double *>(src->pfrac),
+                       static_cast<float *>(dst->pfrac), long(N) * a.nphase);
+    for (int s = 0; s < a.nsolute; ++s) {
+      dst->ccur[s] = 0;
+      hipLaunchKernelGGL(convert_conc_f64_to_mixed, dim3(blocks_for(long(N))), dim3(256), 0,
+                         dst->stream, static_cast<const double *>(src->conc[s][src->ccur[s]]),
+                         static_cast<float2 *>(dst->conc[s][0]), long(N));
+    }
+    HIPCHK(dst, hipGetLastError());
+  } else {
+    HIPCHK(dst, hipMemcpyAsync(dst->phi[0], src->phi[src->cur], N * a.depth * 8,
+                               hipMemcpyDeviceToDevice, dst->stream));
+    HIPCHK(dst, hipMemcpyAsync(dst->pfrac, src->pfrac, N * a.nphase * 8, hipMemcpyDeviceToDevice,
+                               dst->stream));
+    for (int s = 0; s < a.nsolute; ++s) {
+      dst->ccur[s] = 0;
+      HIPCHK(dst, hipMemcpyAsync(dst->conc[s][0], src->conc[s][src->ccur[s]], N * 8,
+                                 hipMemcpyDeviceToDevice, dst->stream));
+    }
+  }
+  HIPCHK(dst, hipStreamSynchronize(dst->stream));
+  for (int ph = 0; ph < 4; ++ph) dst->code_to_id[ph] = src->code_to_id[ph];
+  dst->have_state = true;
+  return PF_OK;
+}
+
+// ---- the reference's accelerator API (gpu_solver.h:18-23) ------------------------------------
+// One process-wide domain, as the reference keeps static device buffers.  value_phase carries all
+// three phases; the stencil always has six faces (the z faces of a one-cell-thick domain meet the
+// cell itself and carry no flux), as in Diff3D (gpu_solver.cpp:160-188).
+static pf_handle *g_compat = nullptr;
+static int g_compat_dims = 3;
+
+int GPU_Init(const int x, const int y, const int z) {
+  if (g_compat) {
+    pf_destroy(g_compat);
+    g_compat = nullptr;
+  }
+  pf_desc d{};
+  d.nx = x;
+  d.ny = y;
+  d.nz = z;
+  d.depth = 4;
+  d.nphase = 3;
+  d.nsolute = 1;
+  d.precision = PF_FP64;
+  d.bc_z = PF_BC_PERIODIC;
+  d.device = 0;
+  if (y == 1 && z == 1) return 0;  // one-dimensional domains fall back to the caller's CPU path
+  if (pf_create(&d, &g_compat) != PF_OK) {
+    std::fprintf(stderr, "GPU_Init: %s\n", pf_last_error(nullptr));
+    g_compat = nullptr;
+    return 0;
+  }
+  g_compat_dims = z > 1 ? 3 : 2;  // gpu_solver.cpp:93-94
+  return 1;
+}
+
+void GPU_Diffusion_solver(double *hdiffusivity, double *hpartition_k, double *hpartition_c,
+                          double *hvalue_phase, double *htotal_conc, int x, int y, int z,
+                          double dt, double dx) {
+  pf_handle *h = g_compat;
+  if (!h || h->d.nx != x || h->d.ny != y || h->d.nz != z) {
+    std::fprintf(stderr, "GPU_Diffusion_solver: GPU_Init was not called for this shape\n");
+    return;
+  }
+  pf_params p{};
+  p.dx = dx;
+  p.eta = dx;
+  p.dims = h->dims;
+  p.pfm_dt = dt;
+  for (int i = 0; i < 3; ++i) {
+    p.diffusivity[0][i] = hdiffusivity[i];
+    for (int j = 0; j < 3; ++j) {
+      p.part_k[0][i][j] = hpartition_k[i + 3 * j];  // column-major (i,j) at i+3j
+      p.part_c[0][i][j] = hpartition_c[i + 3 * j];
+    }
+  }
+  // gpu_solver.cpp:205-209, including the single-precision 0.2f
+  const double max_diff = *std::max_element(hdiffusivity, hdiffusivity + 3);
+  const double del_t = dx * dx / max_diff / g_compat_dims * 0.2f;
+  const int num_loop = int(std::ceil(dt / del_t));
+  const double dt_dxdx = dt / num_loop / (dx * dx);
+  p.n_sub[0] = num_loop;
+  p.dt_d[0] = dt / num_loop;
+  h->p = p;
+  h->have_params = true;
+  const long N = h->N;
+  std::vector<int32_t> idx(size_t(N) * 4, 0);
+  std::vector<double> val(size_t(N) * 4, 0.0);
+  pf_host_state s{};
+  s.index = idx.data();
+  s.value = val.data();
+  s.phase = hvalue_phase;
+  s.conc = htotal_conc;
+  if (pf_upload_state(h, &s) != PF_OK) {
+    std::fprintf(stderr, "GPU_Diffusion_solver: %s\n", h->err.c_str());
+    return;
+  }
+  h->NF = 6;  // Diff3D always sums six faces
+  const int rc = diffuse_any(h, 0, num_loop, dt_dxdx);
+  h->NF = 2 * h->dims;
+  if (rc != PF_OK) {
+    std::fprintf(stderr, "GPU_Diffusion_solver: %s\n", h->err.c_str());
+    return;
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess ||
+      hipMemcpy(htotal_conc, h->conc[0][h->ccur[0]], size_t(N) * 8, hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    std::fprintf(stderr, "GPU_Diffusion_solver: copy back failed\n");
+}
+
+void GPU_Cleanup(void) {
+  if (g_compat) pf_destroy(g_compat);
+  g_compat = nullptr;
+}
+
+}  // extern "C"

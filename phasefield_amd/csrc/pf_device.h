@@ -1,0 +1,97 @@
+// pf_device.h — device-side types shared by the gfx950 kernels.
+//
+// Native HBM layout (one domain, N = nx*ny*nz cells, cell n = x + nx*(y + ny*z), x contiguous):
+//   slot ids    uint16 [N][KI]   KI = 4 for K = 4, 8 for K = 6 (padded, 16-byte rows)
+//   slot phi    T      [N][K]    double (FP64 form) or float (mixed form), double-buffered
+//   phase frac  T      [N][P]    value_phase, P = 2 or 3 phases carried
+//   conc        C      [S][N]    per solute, double-buffered (Jacobi sub-steps):
+//                                C = double (FP64) or float2 {value, Kahan compensation} (mixed)
+// A slot id is a 16-bit grain code: bits 15..14 = phase (1..3), bits 13..0 = grain serial;
+// 0 = empty slot.  The host maps the reference's int32 grain ids (100000+n, 200000+g) to codes
+// and back; the step only ever compares codes for equality, so results do not depend on it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pfamd {
+
+constexpr double kPi = 3.1415926;  // the reference's truncated pi (constant.h:7)
+constexpr int kStencilMax = 10;    // union capacity (solver.cpp:148)
+
+__host__ __device__ inline int code_phase(uint32_t code) { return int(code >> 14); }
+
+// Storage/arithmetic forms.
+struct FormF64 {
+  using T = double;    // phi, value_phase
+  using A = double;    // pair terms, partition, diffusion flux
+  using C = double;    // stored total concentration
+  static constexpr bool kMixed = false;
+};
+struct FormMixed {
+  using T = float;
+  using A = float;
+  using C = float2;    // {c, Kahan compensation}; effective value c - comp
+  static constexpr bool kMixed = true;
+};
+
+template <class C> __device__ inline double conc_value(C c);
+template <> __device__ inline double conc_value<double>(double c) { return c; }
+template <> __device__ inline double conc_value<float2>(float2 c) {
+  return double(c.x) - double(c.y);
+}
+template <class C> __device__ inline auto conc_hi(C c);
+template <> __device__ inline auto conc_hi<double>(double c) { return c; }
+template <> __device__ inline auto conc_hi<float2>(float2 c) { return c.x; }
+
+// periodic wrap of i+d for |d| <= n (Get_neighbor, phase_field.h:58-68, 0-based)
+__device__ inline int wrap(int i, int n) {
+  i = i < 0 ? i + n : i;
+  return i >= n ? i - n : i;
+}
+
+// Step constants for the phase-field sweep (1-based phase tables; row/column 0 = empty slot).
+template <class A> struct PFConsts {
+  A M[4][4], sig[4][4], ent[4][4], eqc[4][4], slope[4][4];
+  A pk[4][4], pc[4][4];  // carbon partition (local_conc of Driving_force)
+  A pkS[4][4], pcS[4][4];  // partition of the second solute (re-partition after the sweep)
+  A f1, f2, f3;
+  double dt, cutoff;
+};
+
+// Step constants for one solute's diffusion sub-step (0-based phase tables).
+template <class A> struct DiffConsts {
+  A pk[3][3], pc[3][3];
+  A D[3];
+  A r;  // dt_d / (dx*dx)
+};
+
+// Slot-row storage of the ids: KI 16-bit codes per cell.
+template <int K> struct IdRowT;
+template <> struct IdRowT<4> {
+  static constexpr int KI = 4;
+  using V = uint2;  // 8 bytes
+};
+template <> struct IdRowT<6> {
+  static constexpr int KI = 8;
+  using V = uint4;  // 16 bytes (2 padding codes)
+};
+
+template <int K> __device__ inline void load_ids(const uint16_t *base, long cell, uint32_t (&g)[K]) {
+  using V = typename IdRowT<K>::V;
+  const V v = reinterpret_cast<const V *>(base)[cell];
+  const uint32_t *w = reinterpret_cast<const uint32_t *>(&v);
+#pragma unroll
+  for (int s = 0; s < K; ++s) g[s] = (w[s >> 1] >> ((s & 1) * 16)) & 0xffffu;
+}
+template <int K> __device__ inline void store_ids(uint16_t *base, long cell, const uint32_t (&g)[K]) {
+  using V = typename IdRowT<K>::V;
+  V v;
+  uint32_t *w = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+  for (int q = 0; q < int(sizeof(V) / 4); ++q) w[q] = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) w[s >> 1] |= (g[s] & 0xffffu) << ((s & 1) * 16);
+  reinterpret_cast<V *>(base)[cell] = v;
+}
+
+}  // namespace pfamd

@@ -1,0 +1,73 @@
+// pf_kernels.h — declarations of the gfx950 kernels (definitions and explicit instantiations in
+// pf_kernels.hip) for the host dispatch in pf_api.hip.
+#pragma once
+#include "pf_device.h"
+
+namespace pfamd {
+
+constexpr int kDiffTY = 8;     // rows of the diffusion tile (64 x kDiffTY threads)
+constexpr int kSweepBS = 128;  // threads per phase-field sweep block (one cell each)
+
+template <class F, int K, int P, int NF, bool NOFLUX, bool ISO>
+__global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
+                                const typename F::T *__restrict__ phi1,
+                                uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
+                                typename F::T *__restrict__ pfrac,
+                                const typename F::C *__restrict__ conc0,
+                                const PFConsts<typename F::A> cs, const int nx, const int ny,
+                                const int nz, const long ncell);
+
+template <class F, int P, int NF, bool NOFLUX>
+__global__ void diffuse_kernel(const typename F::C *__restrict__ cin,
+                               typename F::C *__restrict__ cout,
+                               const typename F::T *__restrict__ pfrac,
+                               const DiffConsts<typename F::A> dc, const int nx, const int ny,
+                               const int nz, const int zchunk, const int tiles_x,
+                               const int tiles_y, const int nblocks);
+
+struct RedPartial {
+  double mass[2], pf[3], cmax[2], cmin[2], serr;
+  long long nif, nnf;
+};
+
+template <class F, int K, int P>
+__global__ void reduce_stage1(const uint16_t *__restrict__ id,
+                              const typename F::T *__restrict__ phi,
+                              const typename F::T *__restrict__ pfrac,
+                              const typename F::C *__restrict__ c0,
+                              const typename F::C *__restrict__ c1, int ns, long ncell,
+                              long per_block, RedPartial *__restrict__ out);
+__global__ void reduce_stage2(RedPartial *__restrict__ part, int nparts);
+
+template <int K, int P>
+__global__ void voronoi_kernel(const double *__restrict__ seeds, int G, double reduction, int nx,
+                               int ny, int nz, long ncell, uint16_t *__restrict__ id,
+                               double *__restrict__ phi, double *__restrict__ pfrac);
+template <int P>
+__global__ void construct_conc_kernel(const double *__restrict__ pfrac, double c1, double c2,
+                                      double c3, long ncell, double *__restrict__ c);
+template <int K, int P>
+__global__ void nucleation_flag_kernel(const uint16_t *__restrict__ id,
+                                       const double *__restrict__ pfrac,
+                                       const double *__restrict__ c0, const DiffConsts<double> dc,
+                                       uint64_t seed_key, long ncell, float *__restrict__ flag,
+                                       uint8_t *__restrict__ mark);
+__global__ void argmax_stage1(const float *__restrict__ flag, long ncell, long per_block,
+                              float *__restrict__ pv, long long *__restrict__ pi);
+__global__ void argmax_stage2(float *__restrict__ pv, long long *__restrict__ pi, int nparts);
+template <int K, int P>
+__global__ void nucleus_paint_kernel(const long long *__restrict__ center, int q, float radius,
+                                     int R, int nx, int ny, int nz, int cx_n, int cy_n, int cz_n,
+                                     uint16_t *__restrict__ id, double *__restrict__ phi,
+                                     double *__restrict__ pfrac, float *__restrict__ flag,
+                                     uint8_t *__restrict__ mark);
+__global__ void nucleus_conc_kernel(const uint8_t *__restrict__ mark, double value, long ncell,
+                                    double *__restrict__ c);
+__global__ void solute_noise_kernel(double mean, double sd, uint64_t seed_key, long ncell,
+                                    double *__restrict__ c);
+__global__ void convert_f64_to_f32(const double *__restrict__ src, float *__restrict__ dst,
+                                   long n);
+__global__ void convert_conc_f64_to_mixed(const double *__restrict__ src, float2 *__restrict__ dst,
+                                          long n);
+
+}  // namespace pfamd

@@ -1,0 +1,258 @@
+// pf_sweep_impl.h — the phase-field sweep kernel (definition; instantiated in pf_sweep_f64.hip and
+// pf_sweep_mixed.hip).
+//
+// One thread per cell (Phase_field_solver, C++-openmp-cuda/src/solver.cpp:147-304):
+//  1. active set: own slots with phi > cutoff at their slot position; then the slots of the
+//     neighbours (-x,+x,-y,+y,-z,+z; first 2*dims faces; a zero id ends a neighbour's list) are
+//     matched by id (first match) or appended while fewer than 10 entries exist.
+//  2. a single entry copies the slots unchanged (value_phase untouched).
+//  3. per entry a: sum over b != a (phases not 1<->3) of
+//       M_ab * ( sigma_ab*[(S_a phi_b - S_b phi_a)/dx^2 + pi^2/(2 eta^2)(phi_a - phi_b)]
+//                + pi/eta sqrt(phi_a phi_b) dG_ab ),
+//     Euler step, clamp to [0,1], zero below cutoff.
+//  4. selection sort (descending, first maximum wins), keep K, value_phase, renormalise by
+//     multiplying with 1/sum.
+// ISO variant (Iso_growth, MicrostructureConstruction.cpp:130-248): neighbour entries are used
+// only when phi > cutoff, the pair term has no sigma/M/driving force, the Euler factor is
+// 0.5/dims*dx*dx (passed as dt) and no phase pair is skipped.
+// The mixed form accumulates the per-entry update, clamps, prunes, sorts and renormalises in FP64.
+//
+// The per-thread union lists (ids, own phi, neighbour sums, updated values) live in LDS as
+// [entry][thread] arrays so the data-dependent match/append/sort indices stay cheap.
+#pragma once
+#include "pf_kernels.h"
+
+namespace pfamd {
+
+// table value for the phase pair (pa, pb), 1-based, zero when either phase is 0 (empty slot)
+template <class A, int P>
+__device__ inline A pair_pick(const A (&tab)[4][4], int pa, int pb) {
+  A r = A(0);
+#pragma unroll
+  for (int i = 1; i <= P; ++i)
+#pragma unroll
+    for (int j = 1; j <= P; ++j)
+      if (pa == i && pb == j) r = tab[i][j];
+  return r;
+}
+
+template <class F, int K, int P, int NF, bool NOFLUX, bool ISO>
+__global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
+    const uint16_t *__restrict__ id1, const typename F::T *__restrict__ phi1,
+    uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
+    typename F::T *__restrict__ pfrac, const typename F::C *__restrict__ conc0,
+    const PFConsts<typename F::A> cs, const int nx, const int ny, const int nz,
+    const long ncell) {
+  using T = typename F::T;
+  using A = typename F::A;
+  __shared__ uint32_t sLg[kStencilMax][kSweepBS];
+  __shared__ T sV1[kStencilMax][kSweepBS];
+  __shared__ T sV2[kStencilMax][kSweepBS];
+  __shared__ double sDel[kStencilMax][kSweepBS];
+  const int t = threadIdx.x;
+  const long cell = long(blockIdx.x) * kSweepBS + t;
+  if (cell >= ncell) return;  // no block-wide barrier below
+  const int x = int(cell % nx);
+  const long yz = cell / nx;
+  const int y = int(yz % ny);
+  const int z = int(yz / ny);
+  const double cutoff = cs.cutoff;
+
+  for (int e = 0; e < kStencilMax; ++e) {
+    sLg[e][t] = 0;
+    sV1[e][t] = T(0);
+    sV2[e][t] = T(0);
+  }
+  uint32_t g0[K];
+  T v0[K];
+  load_ids<K>(id1, cell, g0);
+#pragma unroll
+  for (int s = 0; s < K; ++s) v0[s] = phi1[cell * K + s];
+  int n = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (double(v0[s]) > cutoff) {
+      ++n;
+      sLg[s][t] = g0[s];
+      sV1[s][t] = v0[s];
+    }
+  }
+
+  long nb[6];
+  {
+    const long plane = long(nx) * ny;
+    const long row = long(nx) * y + plane * z;
+    nb[0] = wrap(x - 1, nx) + row;
+    nb[1] = wrap(x + 1, nx) + row;
+    nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
+    nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
+    int zm, zp;
+    if (NOFLUX) {
+      zm = z == 0 ? z : z - 1;
+      zp = z == nz - 1 ? z : z + 1;
+    } else {
+      zm = wrap(z - 1, nz);
+      zp = wrap(z + 1, nz);
+    }
+    nb[4] = x + long(nx) * y + plane * zm;
+    nb[5] = x + long(nx) * y + plane * zp;
+  }
+
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    uint32_t gn[K];
+    load_ids<K>(id1, nb[f], gn);
+    bool live = true;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const uint32_t g = gn[s];
+      live = live && (g != 0);
+      if (live) {
+        const T v = phi1[nb[f] * K + s];
+        if (!ISO || double(v) > cutoff) {
+          int kk = 0;
+          for (; kk < kStencilMax; ++kk)
+            if (sLg[kk][t] == g) break;
+          if (kk < kStencilMax) {
+            sV2[kk][t] += v;
+          } else if (n < kStencilMax) {
+            sLg[n][t] = g;
+            sV2[n][t] += v;
+            ++n;
+          }
+        }
+      }
+    }
+  }
+
+  if (n == 1) {
+    using V = typename IdRowT<K>::V;
+    reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
+#pragma unroll
+    for (int s = 0; s < K; ++s) phi2[cell * K + s] = v0[s];
+    return;
+  }
+
+  // local_conc (value_conc1 of this cell before the update) and the driving force per phase
+  // pair (Driving_force, solver.cpp:374-387: only the concentration of the first phase enters)
+  A dG[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dG[a][b] = A(0);
+  if (!ISO) {
+    A ph[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) ph[j] = A(pfrac[cell * P + j]);
+    const A c = A(conc_hi(conc0[cell]));
+    A lc[4];
+    lc[0] = A(0);
+#pragma unroll
+    for (int i = 1; i <= P; ++i) {
+      A t1 = A(0), t2 = A(0);
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        t1 += ph[j] * cs.pk[i][j + 1];
+        t2 += ph[j] * cs.pc[i][j + 1];
+      }
+      lc[i] = (c - t2) / t1;
+    }
+#pragma unroll
+    for (int a = 1; a <= P; ++a)
+#pragma unroll
+      for (int b = 1; b <= P; ++b) {
+        A df = A(0);
+        if (a != b) df += cs.ent[a][b] * (lc[a] - cs.eqc[a][b]) / cs.slope[a][b];
+        dG[a][b] = df;
+      }
+  }
+
+  for (int a = 0; a < kStencilMax; ++a) sDel[a][t] = 0.0;
+  for (int a = 0; a < n; ++a) {
+    const int pa = code_phase(sLg[a][t]);
+    const A v1a = A(sV1[a][t]), v2a = A(sV2[a][t]);
+    double acc = 0.0;
+    for (int b = 0; b < n; ++b) {
+      if (b == a) continue;
+      const A v1b = A(sV1[b][t]), v2b = A(sV2[b][t]);
+      if (ISO) {
+        const A tt = (v2a * v1b - v2b * v1a) * cs.f1 + (v1a - v1b) * cs.f2;
+        acc += double(tt);
+      } else {
+        const int pb = code_phase(sLg[b][t]);
+        if (pa - pb == 2 || pb - pa == 2) continue;
+        A tt = (v2a * v1b - v2b * v1a) * cs.f1;
+        tt += (v1a - v1b) * cs.f2;
+        tt *= pair_pick<A, P>(cs.sig, pa, pb);
+        tt += cs.f3 * sqrt(v1a * v1b) * pair_pick<A, P>(dG, pa, pb);
+        tt *= pair_pick<A, P>(cs.M, pa, pb);
+        acc += double(tt);
+      }
+    }
+    acc *= cs.dt;
+    acc += double(v1a);
+    if (acc > 1.0)
+      acc = 1.0;
+    else if (acc < cutoff)
+      acc = 0.0;
+    sDel[a][t] = acc;
+  }
+
+  // Sort_xyz (solver.cpp:426-432): only the first K positions are read afterwards, and the
+  // first K passes of the selection sort fix them.
+  for (int a = 0; a < K; ++a) {
+    int m = a;
+    double best = sDel[a][t];
+    for (int b = a + 1; b < kStencilMax; ++b) {
+      const double v = sDel[b][t];
+      if (v > best) {
+        best = v;
+        m = b;
+      }
+    }
+    if (m != a) {
+      sDel[m][t] = sDel[a][t];
+      sDel[a][t] = best;
+      const uint32_t tg = sLg[a][t];
+      sLg[a][t] = sLg[m][t];
+      sLg[m][t] = tg;
+    }
+  }
+
+  double sum = 0.0, vp[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t go[K];
+  double out[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const double d = sDel[s][t];
+    if (d > cutoff) {
+      go[s] = sLg[s][t];
+      out[s] = d;
+      const int p = code_phase(go[s]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q == p) vp[q] += d;
+      sum += d;
+    } else {
+      go[s] = 0;
+      out[s] = 0.0;
+    }
+  }
+  const double inv = 1. / sum;
+  store_ids<K>(id2, cell, go);
+#pragma unroll
+  for (int s = 0; s < K; ++s) phi2[cell * K + s] = T(out[s] * inv);
+#pragma unroll
+  for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
+}
+
+#define PF_INST_SWEEP(F, K, P, NF, NOF, ISO)                                    \
+  template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO>(              \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *, \
+      const PFConsts<F::A>, int, int, int, long);
+#define PF_INST_SWEEP_ALL(F, K, P, ISO)   \
+  PF_INST_SWEEP(F, K, P, 4, false, ISO)   \
+  PF_INST_SWEEP(F, K, P, 6, false, ISO)   \
+  PF_INST_SWEEP(F, K, P, 6, true, ISO)
+
+}  // namespace pfamd

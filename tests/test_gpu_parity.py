@@ -1,0 +1,237 @@
+"""GPU parity: the HIP kernels (through the C-ABI) against the CPU oracle on identical inputs.
+
+FP64 form: bitwise identical active-set maps, phi, value_phase and concentrations (the build has no
+FMA contraction and gfx950's division and square root round correctly), which is stricter than the
+1e-10 relative bar.  Mixed form: against the oracle's mixed form, bitwise as well.
+Shapes include partial 64-wide x tiles, partial y tiles and z-chunk edges.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from phasefield_amd import Domain, gpu_cleanup, gpu_diffusion_solver, gpu_init
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def base_params(tmp_path_factory):
+    m = helpers.small_model(str(tmp_path_factory.mktemp("gp")), "fec_3d_256", nx=16, ny=16,
+                            nz=16, grains=4, nuclei=2)
+    return helpers.model_params(m)
+
+
+def with_phase3(p):
+    """Give the placeholder phase nonzero interface energy/mobility so P=3 paths do work."""
+    q = oracle.as_orc_params(p)
+    for (i, j) in ((1, 2), (2, 1), (2, 2)):
+        q.sigma[i][j] = 0.3e-12
+        q.mobility[i][j] = 0.5e12
+    q.diffusivity[0][2] = 5.0
+    return q
+
+
+def assert_same(gs, os_, K, tag, tol=0.0):
+    assert np.array_equal(gs["index"], os_["index"]), f"{tag}: active-set maps differ"
+    for key in ("value", "phase", "conc"):
+        diff = np.max(np.abs(gs[key] - os_[key])) if gs[key].size else 0.0
+        assert diff <= tol, f"{tag}: {key} differs by {diff}"
+
+
+def run_pair(shape, K, P, S, bc, form, state, params, fn_gpu, fn_orc):
+    nx, ny, nz = shape
+    g = Domain(nx, ny, nz, depth=K, nphase=P, nsolute=S,
+               precision="mixed" if form == oracle.ORC_MIXED else "fp64", bc_z=bc)
+    pp = oracle.as_orc_params(params)
+    pp.dims = 3 - (nz == 1) - (ny == 1)
+    from phasefield_amd._capi import PfParams
+    gp = PfParams.from_buffer_copy(pp)
+    g.set_params(gp)
+    g.upload(*state)
+    o = oracle.OracleDomain(nx, ny, nz, K, S, form, bc)
+    o.set_params(pp)
+    o.set_state(*state)
+    fn_gpu(g, gp)
+    fn_orc(o, pp)
+    gs = g.download()
+    os_ = o.get_state()
+    if form == oracle.ORC_MIXED:
+        gs["conc"] = gs["conc"]  # stored value; compensation compared separately
+        assert np.array_equal(gs["conc_comp"], os_["conc_comp"])
+    g.close()
+    return gs, os_
+
+
+SHAPES = [((32, 32, 1), 0), ((70, 13, 19), 0), ((64, 16, 40), 0), ((40, 24, 18), 1)]
+
+
+@pytest.mark.parametrize("shape,bc", SHAPES)
+@pytest.mark.parametrize("P", [2, 3])
+def test_diffusion_substeps_bitwise(shape, bc, P, base_params, rng):
+    banner(f"diffusion {shape} bc={bc} P={P}")
+    K = 4
+    params = with_phase3(base_params) if P == 3 else base_params
+    st = helpers.random_state(rng, *shape, K, nphase=P)
+    gs, os_ = run_pair(shape, K, P, 1, bc, oracle.ORC_FP64, st, params,
+                       lambda g, p: g.diffuse(0, 37, p.dt_d[0]),
+                       lambda o, p: o.diffuse(0, 37, p.dt_d[0]))
+    assert_same(gs, os_, K, "diffusion")
+
+
+@pytest.mark.parametrize("shape,bc", SHAPES)
+@pytest.mark.parametrize("K,P", [(4, 2), (6, 2), (4, 3), (6, 3)])
+def test_phase_field_sweep_bitwise(shape, bc, K, P, base_params, rng):
+    banner(f"sweep {shape} bc={bc} K={K} P={P}")
+    params = with_phase3(base_params) if P == 3 else base_params
+    st = helpers.random_state(rng, *shape, K, nphase=P)
+    gs, os_ = run_pair(shape, K, P, 1, bc, oracle.ORC_FP64, st, params,
+                       lambda g, p: g.sweep(), lambda o, p: o.sweep())
+    assert_same(gs, os_, K, "sweep")
+
+
+def _setup_state(tmp_path, **over):
+    m = helpers.small_model(str(tmp_path), over.pop("workload", "fec_3d_256"), **over)
+    p = helpers.model_params(m)
+    od = helpers.oracle_initial_state(m, p)
+    s = od.get_state()
+    return m, p, (s["index"], s["value"], s["phase"], s["conc"])
+
+
+@pytest.mark.parametrize("shape,bc,K", [((70, 20, 18), "periodic", 4), ((48, 40, 1), "periodic", 6),
+                                        ((40, 36, 24), "noflux", 4)])
+def test_full_steps_bitwise(tmp_path, shape, bc, K):
+    """Several complete Update() steps (sweep + 200 sub-steps) from a seeded microstructure."""
+    banner(f"full steps {shape} {bc} K={K}")
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=6, nuclei=4, depth=K, bc_z=bc)
+    bcn = 1 if bc == "noflux" else 0
+    gs, os_ = run_pair(shape, K, 2, 1, bcn, oracle.ORC_FP64, st, p,
+                       lambda g, q: g.step(3), lambda o, q: o.step(3))
+    assert_same(gs, os_, K, "full step")
+
+
+def test_two_solutes_bitwise(tmp_path):
+    """Fe-C-Mn: carbon sub-cycled, manganese one explicit update per step."""
+    banner("two solutes")
+    m, p, st = _setup_state(tmp_path, workload="fecmn_3d_512", nx=36, ny=28, nz=20, grains=5,
+                            nuclei=3, precision="fp64")
+    gs, os_ = run_pair((36, 28, 20), 4, 2, 2, 0, oracle.ORC_FP64, st, p,
+                       lambda g, q: g.step(2), lambda o, q: o.step(2))
+    assert_same(gs, os_, 4, "two solutes")
+
+
+def test_mixed_form_matches_oracle_mixed(tmp_path):
+    banner("mixed form")
+    m, p, st = _setup_state(tmp_path, workload="fecmn_3d_512", nx=40, ny=24, nz=20, grains=5,
+                            nuclei=3)
+    gs, os_ = run_pair((40, 24, 20), 4, 2, 2, 0, oracle.ORC_MIXED, st, p,
+                       lambda g, q: g.step(2), lambda o, q: o.step(2))
+    assert_same(gs, os_, 4, "mixed")
+
+
+def test_device_setup_matches_oracle(tmp_path):
+    """Voronoi + Iso_growth + partition + nucleation + Mn noise on the device vs the oracle."""
+    banner("device setup")
+    m = helpers.small_model(str(tmp_path), "fecmn_3d_512", nx=48, ny=40, nz=24, grains=9,
+                            nuclei=6, precision="fp64")
+    p = helpers.model_params(m)
+    od = helpers.oracle_initial_state(m, p)
+    d = m.desc()
+    g = Domain.from_desc(d)
+    g.set_params(p)
+    ic = [m.get_real(f"init_conc{i}") for i in range(3)] + [m.get_real("mn_conc")] * 3
+    g.construct(m.voronoi_seeds(), m.get_real("reduction"), m.get_int("num_init_iterations"), ic)
+    g.solute_noise(1, m.get_real("mn_conc"), m.get_real("mn_noise"), m.get_int("mn_seed"))
+    g.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"),
+               (m.get_real("nucleus_conc"), m.get_real("mn_nucleus_conc")),
+               m.get_int("nucleation_seed"))
+    assert_same(g.download(), od.get_state(), d.depth, "setup")
+
+
+def test_reduce_matches_oracle(tmp_path):
+    banner("reduce")
+    m, p, st = _setup_state(tmp_path, nx=50, ny=30, nz=20, grains=5, nuclei=3)
+    g = Domain(50, 30, 20, depth=4)
+    g.set_params(p)
+    g.upload(*st)
+    o = oracle.OracleDomain(50, 30, 20, 4, 1)
+    o.set_params(p)
+    o.set_state(*st)
+    rg, ro = g.reduce(), o.reduce()
+    assert abs(rg["mass"][0] - ro["mass"][0]) <= 1e-13 * ro["mass"][0]
+    for i in range(3):
+        assert abs(rg["phase_frac"][i] - ro["phase_frac"][i]) <= 1e-14
+    assert rg["max_conc"][0] == ro["max_conc"][0] and rg["min_conc"][0] == ro["min_conc"][0]
+    assert rg["n_interface"] == ro["n_interface"]
+    assert abs(rg["max_sum_err"] - ro["max_sum_err"]) <= 1e-15
+
+
+def test_roundtrips_and_checkpoint(tmp_path):
+    banner("round trips")
+    m, p, st = _setup_state(tmp_path, nx=30, ny=20, nz=10, grains=4, nuclei=2)
+    g = Domain(30, 20, 10, depth=4)
+    g.set_params(p)
+    g.upload(*st)
+    s = g.download(phase_conc=True)
+    assert np.array_equal(s["index"], st[0]) and np.array_equal(s["value"], st[1])
+    assert np.array_equal(s["phase"], st[2]) and np.array_equal(s["conc"], st[3])
+    o = oracle.OracleDomain(30, 20, 10, 4, 1)
+    o.set_params(p)
+    o.set_state(*st)
+    assert np.array_equal(s["phase_conc"], o.get_state()["phase_conc"])
+    blob = g.checkpoint()
+    g.step(2)
+    a = g.download()
+    g.restore(blob)
+    g.step(2)
+    b = g.download()
+    for k in ("index", "value", "phase", "conc"):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_determinism(tmp_path):
+    banner("determinism")
+    m, p, st = _setup_state(tmp_path, nx=66, ny=18, nz=20, grains=5, nuclei=3)
+    outs = []
+    for _ in range(2):
+        g = Domain(66, 18, 20, depth=4)
+        g.set_params(p)
+        g.upload(*st)
+        g.step(2)
+        outs.append((g.download(), g.reduce()))
+        g.close()
+    for k in ("index", "value", "phase", "conc"):
+        assert np.array_equal(outs[0][0][k], outs[1][0][k])
+    assert outs[0][1] == outs[1][1]
+
+
+def test_compat_shim_matches_oracle(base_params, rng):
+    """GPU_Init / GPU_Diffusion_solver / GPU_Cleanup with host pointers (gpu_solver.h:18-23):
+    sub-step count and dt/dx^2 computed as gpu_solver.cpp:205-209 (including its 0.2f)."""
+    banner("compat shim")
+    nx, ny, nz = 40, 30, 12
+    K = 4
+    ids, val, phase, conc = helpers.random_state(rng, nx, ny, nz, K)
+    p = base_params
+    D = np.array([p.diffusivity[0][i] for i in range(3)])
+    pk = np.array([[p.part_k[0][i][j] for j in range(3)] for i in range(3)])
+    pc = np.array([[p.part_c[0][i][j] for j in range(3)] for i in range(3)])
+    assert gpu_init(nx, ny, nz) == 1
+    c = conc.copy()
+    dt, dx = p.pfm_dt, p.dx
+    gpu_diffusion_solver(D, pk.T.reshape(-1), pc.T.reshape(-1), phase, c, nx, ny, nz, dt, dx)
+    gpu_cleanup()
+    del_t = dx * dx / D.max() / 3 * float(np.float32(0.2))
+    nloop = int(np.ceil(dt / del_t))
+    o = oracle.OracleDomain(nx, ny, nz, K, 1)
+    o.set_params(p)
+    o.set_state(ids, val, phase, conc)
+    o.diffuse_factor(0, nloop, dt / nloop / (dx * dx))
+    assert np.array_equal(c, o.get_state()["conc"])
