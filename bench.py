@@ -1,0 +1,217 @@
+"""Benchmark: cell-updates/s of the multi-phase-field + solute-diffusion step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fecmn_3d_512] [--no-cpu]
+
+One step = phase_field::Update(): one phase-field sweep + 200 carbon diffusion sub-steps (+ one
+manganese update for two-solute workloads), with the per-step parameter update on the host.
+Default workload: 512^3 Fe-C-Mn, 512 Voronoi ferrite grains + 1000 austenite nuclei, K = 4.
+`value` is the mixed form (FP32 storage/arithmetic, Kahan-compensated concentration, FP64
+pruning/renormalisation); the FP64 form runs the same steps from the same seeded state and is
+reported beside it with the mixed form's error against it.  The state is resident in HBM before
+the timed region; the region is bracketed by stream synchronisation.
+The roofline entry is for the dominant kernel (the diffusion sub-step): algorithmic bytes per
+launch (read phase fractions + stored concentration, write concentration) over the mean launch
+time from HIP events on the stepping stream.  cpu_baseline times the oracle's OpenMP path
+(oracle/, a restatement of the reference's CPU loop) on the box's host cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s FP32-mix vs FP64 at stated error; %HBM roofline; adjoint/fwd"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup_fp64(m, p):
+    from phasefield_amd import Domain, PF_FP64
+
+    d = m.desc()
+    d.precision = PF_FP64
+    g = Domain.from_desc(d)
+    g.set_params(p)
+    ic = [m.get_real(f"init_conc{i}") for i in range(3)]
+    if d.nsolute > 1:
+        ic += [m.get_real("mn_conc")] * 3
+    g.construct(m.voronoi_seeds(), m.get_real("reduction"), m.get_int("num_init_iterations"), ic)
+    if d.nsolute > 1 and m.get_real("mn_noise") > 0:
+        g.solute_noise(1, m.get_real("mn_conc"), m.get_real("mn_noise"), m.get_int("mn_seed"))
+    g.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"),
+               (m.get_real("nucleus_conc"), m.get_real("mn_nucleus_conc")),
+               m.get_int("nucleation_seed"))
+    g.synchronize()
+    return g
+
+
+def alg_bytes_per_substep(mixed: bool, nphase: int) -> int:
+    """Compulsory bytes of one carbon diffusion sub-step per cell: read value_phase (P values) and
+    the stored concentration, write the concentration (FP64: 8-byte values; mixed: 4-byte phase
+    fractions and an 8-byte {value, Kahan term} concentration)."""
+    b_phi = 4 if mixed else 8
+    return nphase * b_phi + 8 + 8
+
+
+def timed_run(dom, model, steps: int, warmup: int):
+    """warmup untimed steps, then exactly `steps` timed steps with HIP-event kernel timing."""
+    for _ in range(warmup):
+        dom.set_params(model.update_parameter())
+        dom.step(1)
+    dom.synchronize()
+    dom.timing_reset(True)
+    dom.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dom.set_params(model.update_parameter())
+        dom.step(1)
+    dom.synchronize()
+    wall = time.perf_counter() - t0
+    diff_ms, diff_n = dom.timing(0)
+    pf_ms, pf_n = dom.timing(1)
+    dom.timing_reset(False)
+    return wall, diff_ms, diff_n, pf_ms, pf_n
+
+
+def cpu_baseline(workload: str, tmp: str):
+    """Oracle (OpenMP, reference loop schedules) on a 128^3 sample of the workload with the grain
+    and nucleus densities kept: setup, then one full step timed (sweep + all sub-steps)."""
+    import oracle
+    from phasefield_amd import Model, configs
+
+    over = dict(nx=128, ny=128, nz=128, grains=8, nuclei=16, precision="fp64")
+    m = Model(configs.write_inputs(os.path.join(tmp, "cpu"), workload, **over))
+    p = m.update_parameter()
+    d = m.desc()
+    od = oracle.OracleDomain(d.nx, d.ny, d.nz, d.depth, d.nsolute)
+    od.set_params(p)
+    grains = oracle.tessellation(d.nx, d.ny, d.nz, m.voronoi_seeds(), 0.0)
+    ic = [m.get_real(f"init_conc{i}") for i in range(3)]
+    if d.nsolute > 1:
+        ic += [m.get_real("mn_conc")] * 3
+    od.construct(grains, m.get_int("num_init_iterations"), ic)
+    od.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"), (0.8, -1.0), 1)
+    p2 = m.update_parameter()
+    od.set_params(p2)
+    t0 = time.perf_counter()
+    od.step(1)
+    dt = time.perf_counter() - t0
+    n = d.nx * d.ny * d.nz
+    return {"value": n / dt / 1e9, "unit": "Gcell-updates/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": f"1 full step (1 sweep + {p2.n_sub[0]} carbon sub-steps"
+                      f"{' + 1 Mn update' if d.nsolute > 1 else ''}) of a 128^3 cut of the workload "
+                      f"(8 grains, 16 nuclei), FP64, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="fecmn_3d_512")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU halo design is described in DESIGN.md)"}))
+        return 2
+    from phasefield_amd import Domain, Model, PF_MIXED, configs
+
+    tmp = tempfile.mkdtemp(prefix="pfbench_")
+    wl = configs.WORKLOADS[args.workload]
+    m_mixed = Model(configs.write_inputs(os.path.join(tmp, "mixed"), args.workload,
+                                         precision="mixed"))
+    m_fp64 = Model(configs.write_inputs(os.path.join(tmp, "fp64"), args.workload,
+                                        precision="fp64"))
+    d = m_mixed.desc()
+    N = d.nx * d.ny * d.nz
+    p0 = m_mixed.update_parameter()  # Construct's parameter update (time += pfm_dt)
+    m_fp64.update_parameter()
+    log(f"setup {args.workload} {d.nx}x{d.ny}x{d.nz} on the device")
+    t0 = time.perf_counter()
+    g64 = setup_fp64(m_mixed, p0)
+    log(f"setup {time.perf_counter() - t0:.1f} s")
+    mixed = Domain.from_desc(d)
+    mixed.set_params(p0)
+    mixed.copy_from(g64)
+    st0 = g64.reduce()
+    st0_mixed = mixed.reduce()  # after rounding the FP64 state to FP32
+
+    res = {}
+    for name, dom, model in (("mixed", mixed, m_mixed), ("fp64", g64, m_fp64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        log(f"{name}: {args.warmup} warmup + {args.steps} timed steps")
+        wall, dms, dn, pms, pn = timed_run(dom, model, args.steps, args.warmup)
+        st = dom.reduce()
+        launch_us = dms / dn * 1e3
+        bpc = alg_bytes_per_substep(name == "mixed", d.nphase)
+        achieved = N * bpc / (launch_us * 1e-6) / 1e9
+        res[name] = {"wall": wall, "ms_per_step": wall / args.steps * 1e3,
+                     "value": N * args.steps / wall / 1e9, "diff_us": launch_us,
+                     "diff_launches": dn, "pf_us": pms / pn * 1e3, "stats": st,
+                     "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                                  "kernel": f"diffuse_kernel<{'FormMixed' if name == 'mixed' else 'FormF64'},"
+                                            f"{d.nphase},6>",
+                                  "bytes_per_cell": bpc, "cells_per_launch": N,
+                                  "launch_us": round(launch_us, 2)}}
+        log(f"{name}: {res[name]['ms_per_step']:.2f} ms/step, diffusion {launch_us:.1f} us "
+            f"({achieved:.0f} GB/s alg), sweep {res[name]['pf_us']:.1f} us")
+
+    mx = res["mixed"]
+    out = {
+        "metric": METRIC, "value": round(mx["value"], 5), "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mx["ms_per_step"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Voronoi + nucleation)",
+        "config": {"workload": f"{args.workload}: {d.nx}^3 Fe-C{'-Mn' if d.nsolute > 1 else ''}, "
+                               f"{wl['grains']} Voronoi ferrite grains + {wl['nuclei']} austenite "
+                               f"nuclei (r=4), periodic, K={d.depth}, P={d.nphase}, "
+                               f"{p0.n_sub[0]} carbon sub-steps"
+                               f"{' + 1 Mn update' if d.nsolute > 1 else ''} per step",
+                   "form": "mixed (FP32 + Kahan, FP64 prune/renormalise)",
+                   "nx": d.nx, "ny": d.ny, "nz": d.nz},
+        "roofline": mx["roofline"],
+    }
+    if "fp64" in res:
+        f64 = res["fp64"]
+        s64, smx = f64["stats"], mx["stats"]
+        out["fp64"] = {"value": round(f64["value"], 5), "ms_per_step": round(f64["ms_per_step"], 3),
+                       "dtype": "f64", "roofline": f64["roofline"]}
+        out["mixed_over_fp64"] = round(mx["value"] / f64["value"], 4)
+        fg64, fgmx = s64["phase_frac"][0], smx["phase_frac"][0]
+        out["error_vs_fp64"] = {
+            "after_steps": args.warmup + args.steps,
+            "austenite_fraction_rel": abs(fgmx - fg64) / max(abs(fg64), 1e-300),
+            "max_c_rel": [abs(a - b) / abs(b) for a, b in zip(smx["max_conc"][: d.nsolute],
+                                                            s64["max_conc"][: d.nsolute])],
+            "mass_drift_rel_mixed": [abs(a - b) / b for a, b in zip(smx["mass"][: d.nsolute],
+                                                                    st0_mixed["mass"][: d.nsolute])],
+            "mass_fp32_rounding_of_initial_state_rel": [
+                abs(a - b) / b for a, b in zip(st0_mixed["mass"][: d.nsolute],
+                                               st0["mass"][: d.nsolute])],
+            "mass_drift_rel_fp64": [abs(a - b) / b for a, b in zip(s64["mass"][: d.nsolute],
+                                                                   st0["mass"][: d.nsolute])],
+        }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, OpenMP)")
+        out["cpu_baseline"] = cpu_baseline(args.workload, tmp)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
