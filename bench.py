@@ -121,7 +121,21 @@ def main() -> int:
     ap.add_argument("--workload", default="fecmn_3d_512")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 8}))
+        return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1 "
                                    "(the multi-GPU halo design is described in DESIGN.md)"}))

@@ -55,6 +55,7 @@ struct pf_handle {
   double t_ms[2] = {0, 0};
   int64_t t_n[2] = {0, 0};
   int diff_zchunk_override = 0;
+  int diff_variant = 0;  // 0: column-marching kernel, 1: LDS-tiled kernel
 };
 
 namespace {
@@ -177,10 +178,11 @@ struct DiffGeom {
   int tiles_x, tiles_y, zchunk, nblocks;
 };
 
-DiffGeom diff_geom(const pf_handle *h) {
+// rows: tile height; target: workgroups to aim for (z is split into chunks until reached)
+DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk) {
   DiffGeom g;
   g.tiles_x = (h->d.nx + 63) / 64;
-  g.tiles_y = (h->d.ny + kDiffTY - 1) / kDiffTY;
+  g.tiles_y = (h->d.ny + rows - 1) / rows;
   const int nz = h->d.nz;
   int zc;
   if (h->NF != 6) {
@@ -189,9 +191,9 @@ DiffGeom diff_geom(const pf_handle *h) {
     zc = std::min(h->diff_zchunk_override, nz);
   } else {
     const long tiles = long(g.tiles_x) * g.tiles_y;
-    long nzc = std::max(1L, std::min<long>(nz, (2048 + tiles - 1) / tiles));
+    long nzc = std::max(1L, std::min<long>(nz, (target + tiles - 1) / tiles));
     zc = int((nz + nzc - 1) / nzc);
-    zc = std::max(zc, std::min(nz, 16));
+    zc = std::max(zc, std::min(nz, min_chunk));
   }
   g.zchunk = zc;
   g.nblocks = g.tiles_x * g.tiles_y * ((nz + zc - 1) / zc);
@@ -202,12 +204,21 @@ template <class F, int P, int NF, bool NOF>
 hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &dc) {
   using T = typename F::T;
   using C = typename F::C;
-  const DiffGeom g = diff_geom(h);
   const int a = h->ccur[s], b = 1 - a;
-  hipLaunchKernelGGL((diffuse_kernel<F, P, NF, NOF>), dim3(g.nblocks), dim3(64, kDiffTY), 0,
-                     h->stream, static_cast<const C *>(h->conc[s][a]),
-                     static_cast<C *>(h->conc[s][b]), static_cast<const T *>(h->pfrac), dc,
-                     h->d.nx, h->d.ny, h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+  const C *cin = static_cast<const C *>(h->conc[s][a]);
+  C *cout = static_cast<C *>(h->conc[s][b]);
+  const T *pf = static_cast<const T *>(h->pfrac);
+  if (h->diff_variant == 1) {
+    const DiffGeom g = diff_geom(h, kDiffTY, 2048, 16);
+    hipLaunchKernelGGL((diffuse_kernel<F, P, NF, NOF, kDiffTY>), dim3(g.nblocks),
+                       dim3(64, kDiffTY + 3), 0, h->stream, cin, cout, pf, dc, h->d.nx, h->d.ny,
+                       h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+  } else {
+    const DiffGeom g = diff_geom(h, kMarchBY, 8192, 8);
+    hipLaunchKernelGGL((diffuse_march_kernel<F, P, NF, NOF, kMarchBY>), dim3(g.nblocks),
+                       dim3(64, kMarchBY), 0, h->stream, cin, cout, pf, dc, h->d.nx, h->d.ny,
+                       h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+  }
   h->ccur[s] = b;
   return hipGetLastError();
 }
@@ -314,6 +325,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   h->tsz = d.precision == PF_MIXED ? 4 : 8;
   h->csz = 8;  // double, or float2 {value, compensation}
   if (const char *e = std::getenv("PFAMD_DIFF_ZCHUNK")) h->diff_zchunk_override = std::atoi(e);
+  if (const char *e = std::getenv("PFAMD_DIFF_KERNEL")) h->diff_variant = std::strcmp(e, "lds") == 0;
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {
     if (e == hipSuccess) return PF_OK;

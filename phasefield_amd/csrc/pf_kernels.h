@@ -17,13 +17,23 @@ __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const PFConsts<typename F::A> cs, const int nx, const int ny,
                                 const int nz, const long ncell);
 
-template <class F, int P, int NF, bool NOFLUX>
+template <class F, int P, int NF, bool NOFLUX, int TY>
 __global__ void diffuse_kernel(const typename F::C *__restrict__ cin,
                                typename F::C *__restrict__ cout,
                                const typename F::T *__restrict__ pfrac,
                                const DiffConsts<typename F::A> dc, const int nx, const int ny,
                                const int nz, const int zchunk, const int tiles_x,
                                const int tiles_y, const int nblocks);
+
+constexpr int kMarchBY = 4;  // rows of the column-marching diffusion tile (64 x kMarchBY lanes)
+
+template <class F, int P, int NF, bool NOFLUX, int BY>
+__global__ void diffuse_march_kernel(const typename F::C *__restrict__ cin,
+                                     typename F::C *__restrict__ cout,
+                                     const typename F::T *__restrict__ pfrac,
+                                     const DiffConsts<typename F::A> dc, const int nx,
+                                     const int ny, const int nz, const int zchunk,
+                                     const int tiles_x, const int tiles_y, const int nblocks);
 
 struct RedPartial {
   double mass[2], pf[3], cmax[2], cmin[2], serr;
