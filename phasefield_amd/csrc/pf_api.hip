@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/phasefield_amd.h"
@@ -55,7 +56,11 @@ struct pf_handle {
   double t_ms[2] = {0, 0};
   int64_t t_n[2] = {0, 0};
   int diff_zchunk_override = 0;
-  int diff_variant = 0;  // 0: column-marching kernel, 1: LDS-tiled kernel
+  int diff_variant = 0;  // 0: bulk-aware column kernel, 1: LDS-tiled full-formula kernel
+  uint8_t *flag = nullptr;   // per-cell bulk phase (0 = needs the full formula)
+  bool flag_valid = false;   // flags describe the current value_phase and stencil
+  int flag_nf = 0;           // stencil faces the flags were computed for
+  int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
 };
 
 namespace {
@@ -155,6 +160,7 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
                      static_cast<const C *>(h->conc[0][h->ccur[0]]), cs, h->d.nx, h->d.ny, h->d.nz,
                      h->N);
   h->cur = b;
+  h->flag_valid = false;
   return hipGetLastError();
 }
 
@@ -214,10 +220,31 @@ hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &
                        dim3(64, kDiffTY + 3), 0, h->stream, cin, cout, pf, dc, h->d.nx, h->d.ny,
                        h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
   } else {
-    const DiffGeom g = diff_geom(h, kMarchBY, 8192, 8);
-    hipLaunchKernelGGL((diffuse_march_kernel<F, P, NF, NOF, kMarchBY>), dim3(g.nblocks),
-                       dim3(64, kMarchBY), 0, h->stream, cin, cout, pf, dc, h->d.nx, h->d.ny,
-                       h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+    if (!h->flag_valid || h->flag_nf != NF) {
+      hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(blocks_for(h->N)), dim3(256), 0,
+                         h->stream, pf, h->flag, h->d.nx, h->d.ny, h->d.nz, h->N);
+      h->flag_valid = true;
+      h->flag_nf = NF;
+    }
+    auto go = [&](auto by) {
+      constexpr int BY = decltype(by)::value;
+      // z-chunks of ~16 planes: the measured optimum at 512^3 (4 tile rows, 8192+ workgroups)
+      const DiffGeom g = diff_geom(h, BY, 131072 / BY, 8);
+      hipLaunchKernelGGL((diffuse_bulk_kernel<F, P, NF, NOF, BY>), dim3(g.nblocks), dim3(64, BY),
+                         0, h->stream, cin, cout, pf, h->flag, dc, h->d.nx, h->d.ny, h->d.nz,
+                         g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+    };
+    // below ~64M cells the storage-order kernel measured faster (caches hold more of the domain)
+    const int by = h->diff_by >= 0 ? h->diff_by : (h->N < (1L << 26) ? 0 : 4);
+    if (by == 0)
+      hipLaunchKernelGGL((diffuse_linear_kernel<F, P, NF, NOF>), dim3(blocks_for(h->N)), dim3(256),
+                         0, h->stream, cin, cout, pf, h->flag, dc, h->d.nx, h->d.ny, h->d.nz);
+    else if (by == 16)
+      go(std::integral_constant<int, 16>());
+    else if (by == 8)
+      go(std::integral_constant<int, 8>());
+    else
+      go(std::integral_constant<int, 4>());
   }
   h->ccur[s] = b;
   return hipGetLastError();
@@ -270,6 +297,7 @@ void free_all(pf_handle *h) {
       if (h->conc[s][b]) hipFree(h->conc[s][b]);
   }
   if (h->pfrac) hipFree(h->pfrac);
+  if (h->flag) hipFree(h->flag);
   if (h->red) hipFree(h->red);
   for (int k = 0; k < 2; ++k)
     for (auto &pr : h->ev[k]) {
@@ -325,7 +353,9 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   h->tsz = d.precision == PF_MIXED ? 4 : 8;
   h->csz = 8;  // double, or float2 {value, compensation}
   if (const char *e = std::getenv("PFAMD_DIFF_ZCHUNK")) h->diff_zchunk_override = std::atoi(e);
-  if (const char *e = std::getenv("PFAMD_DIFF_KERNEL")) h->diff_variant = std::strcmp(e, "lds") == 0;
+  if (const char *e = std::getenv("PFAMD_DIFF_KERNEL"))
+    h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
+  if (const char *e = std::getenv("PFAMD_DIFF_BY")) h->diff_by = std::atoi(e);
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {
     if (e == hipSuccess) return PF_OK;
@@ -345,6 +375,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   }
   if ((rc = chk(hipMalloc(&h->pfrac, N * d.nphase * h->tsz), "alloc value_phase")) != PF_OK)
     return rc;
+  if ((rc = chk(hipMalloc(&h->flag, N), "alloc bulk flags")) != PF_OK) return rc;
   h->red_blocks = int(std::min<long>(1024, std::max<long>(1, (h->N + 4095) / 4096)));
   if ((rc = chk(hipMalloc(&h->red, sizeof(RedPartial) * h->red_blocks), "alloc partials")) != PF_OK)
     return rc;
@@ -451,6 +482,7 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
   h->cur = 0;
   h->ccur[0] = h->ccur[1] = 0;
   h->have_state = true;
+  h->flag_valid = false;
   return PF_OK;
 }
 
@@ -730,6 +762,7 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
     p += N * h->csz;
   }
   h->have_state = true;
+  h->flag_valid = false;
   return PF_OK;
 }
 
@@ -792,6 +825,7 @@ int pf_setup_construct(pf_handle *h, const double *seeds, int G, double reductio
   for (int ph = 0; ph < 4; ++ph) h->code_to_id[ph].assign(1, 0);
   for (int g = 1; g <= G; ++g) h->code_to_id[2].push_back(200000 + g);
   h->have_state = true;
+  h->flag_valid = false;
   return PF_OK;
 }
 
@@ -873,6 +907,7 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
   if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("nucleate: ") + hipGetErrorString(e));
   h->code_to_id[1].assign(1, 0);
   for (int q = 1; q <= n_nuclei; ++q) h->code_to_id[1].push_back(100000 + q);
+  h->flag_valid = false;
   return PF_OK;
 }
 
@@ -935,6 +970,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
   HIPCHK(dst, hipStreamSynchronize(dst->stream));
   for (int ph = 0; ph < 4; ++ph) dst->code_to_id[ph] = src->code_to_id[ph];
   dst->have_state = true;
+  dst->flag_valid = false;
   return PF_OK;
 }
 

@@ -7,8 +7,17 @@
 // All floating-point expressions keep the reference's evaluation order; the library is built with
 // -ffp-contract=off, and HIP's division and square root are IEEE correctly rounded on gfx950.
 //
-// Work layout: a workgroup owns a 64 x TY column tile and marches through a z-chunk, one plane per
-// iteration.  It has TY + 3 wavefronts: TY interior rows, the rows above and below the tile, and
+// Kernels in this file:
+//   classify_kernel        once per phase-field sweep: per-cell flag of "bulk" cells whose whole
+//                          stencil sits in one phase (exact reduction of the formula, see below)
+//   diffuse_bulk_kernel    the default sub-step for large domains: column-marching, U planes in
+//                          flight per lane, bulk cells skip the partition and weights
+//   diffuse_linear_kernel  the same arithmetic, one thread per cell in storage order (small domains)
+//   diffuse_kernel         full formula for every cell, LDS-tiled (selected with
+//                          PFAMD_DIFF_KERNEL=lds; kept as an independent path for the parity tests)
+//
+// diffuse_kernel layout: a workgroup owns a 64 x TY column tile and marches through a z-chunk, one
+// plane per iteration.  It has TY + 3 wavefronts: TY interior rows, the rows above and below the tile, and
 // one wavefront for the 2*TY cells left and right of the tile.  Every lane therefore partitions
 // exactly one cell per plane (no wavefront waits on a busier one at the barrier), writes its
 // partitioned concentrations and phase fractions to LDS (double-buffered, one barrier per plane),
@@ -216,17 +225,80 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
   }
 }
 
-// Column-marching variant without LDS or barriers: a lane owns one (x, y) column of a 64 x BY tile
-// and walks a z-chunk.  Its own column's partitions at z-1, z, z+1 stay in registers (one new
-// plane is loaded, two planes ahead); the four x/y neighbours of the current plane are loaded
-// (cache hits: they are other lanes' own cells) and partitioned again by this lane.  Latency is
-// hidden by occupancy instead of software pipelining across a barrier.
+// ------------------------------------------------------------------------------------------------
+// Bulk classification, run once per phase-field sweep (value_phase is constant over the diffusion
+// sub-cycle): flag = p (1-based) when the cell and all 2*dims stencil neighbours hold exactly the
+// unit vector e_p in value_phase, else 0.  For such a cell the reference's sub-step reduces
+// exactly to del = sum_faces (c_nb - c) * D_p:
+//   weights are sqrt(1*1) = 1 for phase p and sqrt(0*0) = 0 for the others, whose terms are
+//   0 * finite = +-0 and leave the running sum unchanged (it starts at +0 and can never become
+//   -0 by round-to-nearest addition); the partition of phase p is (c - t2)/t1 with t1 = 1 and
+//   t2 = +-0, i.e. exactly c.
+template <class T, int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfrac,
+                                                       uint8_t *__restrict__ flag, const int nx,
+                                                       const int ny, const int nz,
+                                                       const long ncell) {
+  const long cell = long(blockIdx.x) * 256 + threadIdx.x;
+  if (cell >= ncell) return;
+  const int x = int(cell % nx);
+  const long yz = cell / nx;
+  const int y = int(yz % ny);
+  const int z = int(yz / ny);
+  auto unit = [&](long c) -> int {  // p if value_phase(c) == e_p exactly, else 0
+    int p = 0, ones = 0, zeros = 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const T v = pfrac[c * P + q];
+      if (v == T(1)) {
+        ++ones;
+        p = q + 1;
+      } else if (v == T(0)) {
+        ++zeros;
+      }
+    }
+    return (ones == 1 && zeros == P - 1) ? p : 0;
+  };
+  int p = unit(cell);
+  if (p != 0) {
+    const long plane = long(nx) * ny;
+    long nb[6];
+    nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
+    nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;
+    nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
+    nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
+    const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+    const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+    nb[4] = x + long(nx) * y + plane * zm;
+    nb[5] = x + long(nx) * y + plane * zp;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+      if (unit(nb[f]) != p) p = 0;
+  }
+  flag[cell] = uint8_t(p);
+}
+
+// Bulk-aware sub-step: a lane owns one (x, y) column of a 64 x BY tile and walks a z-chunk, U
+// planes per iteration with all of their loads issued before any arithmetic (the first touch of a
+// plane misses to HBM; U planes in flight per lane hide that latency).  Bulk cells (flag p) read
+// only the stored concentrations of the stencil and apply del = sum_faces (c_nb - c) * D_p; other
+// cells run the full partition + weighted flux on their seven cells.  No LDS and no barrier:
+// neighbouring lanes and planes hit in cache.  Plane bases are wave-uniform (scalar) pointers and
+// lanes index them with 32-bit column offsets.
+template <class F> __device__ inline typename F::A load_hi(const typename F::C *__restrict__ p,
+                                                          int i) {
+  return typename F::A(conc_hi(p[i]));
+}
+template <> __device__ inline float load_hi<FormMixed>(const float2 *__restrict__ p, int i) {
+  return reinterpret_cast<const float *>(p)[2 * i];
+}
+
 template <class F, int P, int NF, bool NOFLUX, int BY>
-__global__ __launch_bounds__(64 * BY) void diffuse_march_kernel(
+__global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
-    const typename F::T *__restrict__ pfrac, const DiffConsts<typename F::A> dc, const int nx,
-    const int ny, const int nz, const int zchunk, const int tiles_x, const int tiles_y,
-    const int nblocks) {
+    const typename F::T *__restrict__ pfrac, const uint8_t *__restrict__ flag,
+    const DiffConsts<typename F::A> dc, const int nx, const int ny, const int nz,
+    const int zchunk, const int tiles_x, const int tiles_y, const int nblocks) {
   using A = typename F::A;
   using C = typename F::C;
   using R = RawCell<F, P>;
@@ -241,90 +313,188 @@ __global__ __launch_bounds__(64 * BY) void diffuse_march_kernel(
   if (x >= nx || y >= ny) return;  // no barrier in this kernel
   const int z1 = min(z0 + zchunk, nz);
   const long plane = long(nx) * ny;
-  const long c0 = x + long(nx) * y;
-  long cn[4];
-  cn[0] = wrap(x - 1, nx) + long(nx) * y;
-  cn[1] = wrap(x + 1, nx) + long(nx) * y;
-  cn[2] = x + long(nx) * wrap(y - 1, ny);
-  cn[3] = x + long(nx) * wrap(y + 1, ny);
+  const int oc = x + nx * y;  // column offset within a plane (N <= 2^31 cells)
+  int on[4];
+  on[0] = wrap(x - 1, nx) + nx * y;
+  on[1] = wrap(x + 1, nx) + nx * y;
+  on[2] = x + nx * wrap(y - 1, ny);
+  on[3] = x + nx * wrap(y + 1, ny);
   auto zidx = [&](int z) -> int {
     if (NOFLUX) return z < 0 ? 0 : (z >= nz ? nz - 1 : z);
     return z < 0 ? z + nz : (z >= nz ? z - nz : z);
   };
-  auto load = [&](long colm, int z) -> R {
-    R r;
-    const long i = colm + plane * zidx(z);
-    r.c = cin[i];
+  constexpr int U = kDiffUnroll;
+  for (int z = z0; z < z1; z += U) {
+    const int nv = min(U, z1 - z);  // planes of this batch (wave-uniform)
+    uint8_t fl[U];
+    C cc[U];
+    A xy[U][4];
 #pragma unroll
-    for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
-    return r;
-  };
-  CellPF<A> prv, cur, nxt;
-  C ccur;
-  A wmz[P];
-  {
-    const R r0 = load(c0, z0);
-    ccur = r0.c;
-    partition_cell<F, P>(r0, dc, cur);
-    if (NF == 6) {
-      partition_cell<F, P>(load(c0, z0 - 1), dc, prv);
+    for (int u = 0; u < U; ++u) {
+      const long base = plane * min(z + u, z1 - 1);
+      const C *__restrict__ pl = cin + base;
+      fl[u] = flag[base + oc];
+      cc[u] = pl[oc];
 #pragma unroll
-      for (int p = 0; p < P; ++p) wmz[p] = face_weight<A>(prv.ph[p] * cur.ph[p]);
+      for (int f = 0; f < 4; ++f)
+        if (f < NF) xy[u][f] = load_hi<F>(pl, on[f]);
     }
-  }
-  for (int z = z0; z < z1; ++z) {
-    R rn[4];
+    A zlo = A(0), zhi = A(0);
+    if (NF == 6) {
+      zlo = load_hi<F>(cin + plane * zidx(z - 1), oc);
+      zhi = load_hi<F>(cin + plane * zidx(z + nv), oc);
+    }
+    unsigned general = 0;  // planes of this batch whose cell needs the full formula
 #pragma unroll
-    for (int f = 0; f < 4; ++f)
-      if (f < NF) rn[f] = load(cn[f], z);
-    R rz;
-    if (NF == 6) rz = load(c0, z + 1);
-    A del = A(0);
+    for (int u = 0; u < U; ++u) {
+      if (u < nv) {
+        if (fl[u] != 0) {
+          A Dp = dc.D[0];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      if (f < NF) {
+          for (int q = 1; q < P; ++q)
+            if (fl[u] == q + 1) Dp = dc.D[q];
+          const A c = A(conc_hi(cc[u]));
+          A del = A(0);
+#pragma unroll
+          for (int f = 0; f < 4; ++f)
+            if (f < NF) del += (xy[u][f] - c) * Dp;
+          if (NF == 6) {
+            const A m = u == 0 ? zlo : A(conc_hi(cc[u > 0 ? u - 1 : 0]));
+            const A p = u + 1 < nv ? A(conc_hi(cc[u + 1 < U ? u + 1 : u])) : zhi;
+            del += (m - c) * Dp;
+            del += (p - c) * Dp;
+          }
+          del *= dc.r;
+          cout[plane * (z + u) + oc] = conc_update<F>(cc[u], del);
+        } else {
+          general |= 1u << u;
+        }
+      }
+    }
+    // cells next to a phase boundary: the full partition + weighted flux, one plane at a time
+    while (general) {
+      const int u = __builtin_ctz(general);
+      general &= general - 1;
+      const long base = plane * (z + u);
+      long nbi[6];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) nbi[f] = base + on[f];
+      nbi[4] = oc + plane * zidx(z + u - 1);
+      nbi[5] = oc + plane * zidx(z + u + 1);
+      const long i = base + oc;
+      R r;
+      r.c = cin[i];
+#pragma unroll
+      for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
+      CellPF<A> me;
+      partition_cell<F, P>(r, dc, me);
+      A del = A(0);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        R rn;
+        rn.c = cin[nbi[f]];
+#pragma unroll
+        for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
         CellPF<A> nb;
-        partition_cell<F, P>(rn[f], dc, nb);
+        partition_cell<F, P>(rn, dc, nb);
 #pragma unroll
         for (int p = 0; p < P; ++p)
-          del += face_weight<A>(nb.ph[p] * cur.ph[p]) * (nb.c[p] - cur.c[p]) * dc.D[p];
+          del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
       }
-    }
-    C cnext{};
-    if (NF == 6) {
-      cnext = rz.c;
-      partition_cell<F, P>(rz, dc, nxt);
-      A wpz[P];
-#pragma unroll
-      for (int p = 0; p < P; ++p) del += wmz[p] * (prv.c[p] - cur.c[p]) * dc.D[p];
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        wpz[p] = face_weight<A>(nxt.ph[p] * cur.ph[p]);
-        del += wpz[p] * (nxt.c[p] - cur.c[p]) * dc.D[p];
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p) wmz[p] = wpz[p];
-    }
-    del *= dc.r;
-    cout[c0 + plane * z] = conc_update<F>(ccur, del);
-    if (NF == 6) {
-      prv = cur;
-      cur = nxt;
-      ccur = cnext;
-    } else if (z + 1 < z1) {
-      const R r = load(c0, z + 1);
-      ccur = r.c;
-      partition_cell<F, P>(r, dc, cur);
+      del *= dc.r;
+      cout[i] = conc_update<F>(r.c, del);
     }
   }
 }
 
+// Storage-order variant: one thread per cell, workgroups over consecutive cells (sequential HBM
+// streams); the six neighbours come from the caches.  Same arithmetic as diffuse_bulk_kernel.
+template <class F, int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void diffuse_linear_kernel(
+    const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
+    const typename F::T *__restrict__ pfrac, const uint8_t *__restrict__ flag,
+    const DiffConsts<typename F::A> dc, const int nx, const int ny, const int nz) {
+  using A = typename F::A;
+  using R = RawCell<F, P>;
+  const long plane = long(nx) * ny;
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= plane * nz) return;
+  const int x = int(i % nx);
+  const long yz = i / nx;
+  const int y = int(yz % ny);
+  const int z = int(yz / ny);
+  long nbi[6];
+  nbi[0] = i - x + wrap(x - 1, nx);
+  nbi[1] = i - x + wrap(x + 1, nx);
+  nbi[2] = i + long(nx) * (wrap(y - 1, ny) - y);
+  nbi[3] = i + long(nx) * (wrap(y + 1, ny) - y);
+  const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+  const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+  nbi[4] = i + plane * (zm - z);
+  nbi[5] = i + plane * (zp - z);
+  const uint8_t fl = flag[i];
+  const typename F::C cv = cin[i];
+  A del = A(0);
+  if (fl != 0) {
+    A Dp = dc.D[0];
+#pragma unroll
+    for (int q = 1; q < P; ++q)
+      if (fl == q + 1) Dp = dc.D[q];
+    const A c = A(conc_hi(cv));
+    A nv[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) nv[f] = A(conc_hi(cin[nbi[f]]));
+#pragma unroll
+    for (int f = 0; f < NF; ++f) del += (nv[f] - c) * Dp;
+  } else {
+    R r;
+    r.c = cv;
+#pragma unroll
+    for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
+    CellPF<A> me;
+    partition_cell<F, P>(r, dc, me);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      R rn;
+      rn.c = cin[nbi[f]];
+#pragma unroll
+      for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
+      CellPF<A> nb;
+      partition_cell<F, P>(rn, dc, nb);
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
+    }
+  }
+  del *= dc.r;
+  cout[i] = conc_update<F>(cv, del);
+}
+
 // explicit instantiations used by the host dispatch (pf_api.hip)
+#define PF_INST_CLASSIFY(T, P)                                                               \
+  template __global__ void classify_kernel<T, P, 4, false>(const T *, uint8_t *, int, int, int, \
+                                                           long);                              \
+  template __global__ void classify_kernel<T, P, 6, false>(const T *, uint8_t *, int, int, int, \
+                                                           long);                              \
+  template __global__ void classify_kernel<T, P, 6, true>(const T *, uint8_t *, int, int, int,  \
+                                                          long);
+PF_INST_CLASSIFY(double, 2)
+PF_INST_CLASSIFY(double, 3)
+PF_INST_CLASSIFY(float, 2)
+PF_INST_CLASSIFY(float, 3)
 #define PF_INST_DIFF_ONE(F, P, NF, NOF)                                                           \
+  template __global__ void diffuse_linear_kernel<F, P, NF, NOF>(                                  \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int); \
+  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, 4>(                                 \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
+      int, int, int, int);                                                                      \
+  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, 8>(                                 \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
+      int, int, int, int);                                                                      \
+  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, 16>(                                \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
+      int, int, int, int);                                                                      \
   template __global__ void diffuse_kernel<F, P, NF, NOF, kDiffTY>(                                \
-      const F::C *, F::C *, const F::T *, const DiffConsts<F::A>, int, int, int, int, int, int, \
-      int);                                                                                     \
-  template __global__ void diffuse_march_kernel<F, P, NF, NOF, kMarchBY>(                         \
       const F::C *, F::C *, const F::T *, const DiffConsts<F::A>, int, int, int, int, int, int, \
       int);
 #define PF_INST_DIFF(F, P)               \

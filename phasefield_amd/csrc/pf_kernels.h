@@ -25,15 +25,29 @@ __global__ void diffuse_kernel(const typename F::C *__restrict__ cin,
                                const int nz, const int zchunk, const int tiles_x,
                                const int tiles_y, const int nblocks);
 
-constexpr int kMarchBY = 4;  // rows of the column-marching diffusion tile (64 x kMarchBY lanes)
+constexpr int kMarchBY = 8;     // default rows of the bulk diffusion tile (64 x kMarchBY lanes)
+constexpr int kDiffUnroll = 4;  // planes per iteration of a column (loads issued together)
+
+template <class F, int P, int NF, bool NOFLUX>
+__global__ void diffuse_linear_kernel(const typename F::C *__restrict__ cin,
+                                      typename F::C *__restrict__ cout,
+                                      const typename F::T *__restrict__ pfrac,
+                                      const uint8_t *__restrict__ flag,
+                                      const DiffConsts<typename F::A> dc, const int nx,
+                                      const int ny, const int nz);
+
+template <class T, int P, int NF, bool NOFLUX>
+__global__ void classify_kernel(const T *__restrict__ pfrac, uint8_t *__restrict__ flag,
+                                const int nx, const int ny, const int nz, const long ncell);
 
 template <class F, int P, int NF, bool NOFLUX, int BY>
-__global__ void diffuse_march_kernel(const typename F::C *__restrict__ cin,
-                                     typename F::C *__restrict__ cout,
-                                     const typename F::T *__restrict__ pfrac,
-                                     const DiffConsts<typename F::A> dc, const int nx,
-                                     const int ny, const int nz, const int zchunk,
-                                     const int tiles_x, const int tiles_y, const int nblocks);
+__global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,
+                                    typename F::C *__restrict__ cout,
+                                    const typename F::T *__restrict__ pfrac,
+                                    const uint8_t *__restrict__ flag,
+                                    const DiffConsts<typename F::A> dc, const int nx, const int ny,
+                                    const int nz, const int zchunk, const int tiles_x,
+                                    const int tiles_y, const int nblocks);
 
 struct RedPartial {
   double mass[2], pf[3], cmax[2], cmin[2], serr;
