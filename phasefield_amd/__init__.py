@@ -224,6 +224,45 @@ class Domain:
         _check(lib().pf_rex_fraction(self._h, C.byref(v)), self._h, "pf_rex_fraction")
         return v.value
 
+    # --- discrete adjoint --------------------------------------------------------------------
+    def adjoint(self, n_steps: int, sample_steps=(), target_fraction=(), weight_conc: float = 0.0,
+                target_conc: np.ndarray | None = None, checkpoint_every: int = 1,
+                want_conc0: bool = True) -> dict:
+        """Forward `n_steps` steps from the current state with checkpoints, then the reverse sweep.
+
+        Loss = sum_k (f_gamma(sample_steps[k]) - target_fraction[k])^2
+               + weight_conc * ||c_final - target_conc||^2.
+        Returns the loss, the forward f_gamma samples and dL/d(mobility scale),
+        dL/d(interface-energy scale), dL/dD(phase 1, carbon) and dL/dc0 (carbon, per cell).
+        The domain holds the final forward state afterwards.
+        """
+        samples = list(sample_steps)
+        targets = list(target_fraction)
+        if len(samples) != len(targets) or len(samples) > 64:
+            raise ValueError("sample_steps and target_fraction must pair up (at most 64)")
+        sp = _capi.PfAdjointSpec()
+        sp.n_steps = int(n_steps)
+        sp.checkpoint_every = int(checkpoint_every)
+        sp.n_samples = len(samples)
+        for q, (s, t) in enumerate(zip(samples, targets)):
+            sp.sample_steps[q] = int(s)
+            sp.target_fraction[q] = float(t)
+        sp.weight_conc = float(weight_conc)
+        tc = None
+        if target_conc is not None:
+            tc = np.ascontiguousarray(target_conc, np.float64).reshape(-1)
+            if tc.size != self.N:
+                raise ValueError("target_conc must hold one value per cell")
+            sp.target_conc = _dp(tc)
+        res = _capi.PfAdjointResult()
+        g = np.zeros(self.N, np.float64) if want_conc0 else None
+        res.d_conc0 = _dp(g)
+        _check(lib().pf_adjoint_run(self._h, C.byref(sp), C.byref(res)), self._h, "pf_adjoint_run")
+        return {"loss": res.loss, "fraction": list(res.fraction[: len(samples)]),
+                "d_mobility": res.d_mobility, "d_sigma": res.d_sigma,
+                "d_diff_gamma": res.d_diff_gamma, "d_conc0": g,
+                "forward_ms": res.forward_ms, "reverse_ms": res.reverse_ms}
+
 
 class Model:
     """Parsed input (pf_model): the reference's positional main file + thermodynamics file."""

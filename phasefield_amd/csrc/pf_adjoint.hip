@@ -1,11 +1,691 @@
-// pf_adjoint.hip — discrete adjoint of the fused step (gradient harness).
+// pf_adjoint.hip — discrete adjoint of the FP64 step (gradient harness).
+//
+// Forward step k (phase_field::Update, C++-openmp-cuda/src/phase_field.cpp:243-283):
+//   (phi_k, pf_k, c_k) --sweep--> (phi_{k+1}, pf_{k+1}, c_k) --n_sub diffusion sub-steps--> c_{k+1}
+// The adjoint treats the discrete structure of each step (active-set union, clamp and prune
+// decisions, the sort permutation, the top-K cut) as fixed and differentiates the arithmetic on
+// it; sqrt'(0) := 0; dt, n_sub and the sub-step length are held fixed.
+//
+// Kernels:
+//   adj_diffuse_kernel   reverse of one diffusion sub-step (solver.cpp:343-372): a gather stencil
+//                        on the concentration adjoint with the forward weights times dc_p/dc, plus
+//                        the value_phase adjoint (weights and partition) and dL/dD_p partials.
+//   adj_sweep_kernel     reverse of the phase-field sweep of one cell (solver.cpp:147-304):
+//                        recompute the union, pair terms, clamps and sort, then propagate through
+//                        renormalisation -> sort -> clamp -> pair terms -> (own phi, neighbour sums,
+//                        local concentration, mobility and interface-energy scales).
+//   adj_gather_kernel    scatter of the neighbour-sum adjoints back to the slots that fed them, as
+//                        a gather over the stencil (a neighbour slot feeds the entry with its id).
+// Schedule: the forward run keeps a device checkpoint every `checkpoint_every` steps; the reverse
+// sweep recomputes each segment, then per step recomputes and stores the sub-step concentrations
+// and runs the reverse kernels backwards.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
 #include "../../include/phasefield_amd.h"
+#include "pf_handle_access.h"
+#include "pf_kernels.h"
+
+namespace pfamd {
+
+constexpr int kAdjBS = 64;
+
+__device__ inline void nbrs(long cell, int nx, int ny, int nz, bool noflux, long (&nb)[6]) {
+  const int x = int(cell % nx);
+  const long yz = cell / nx;
+  const int y = int(yz % ny);
+  const int z = int(yz / ny);
+  const long plane = long(nx) * ny;
+  nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
+  nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;
+  nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
+  nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
+  const int zm = noflux ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+  const int zp = noflux ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+  nb[4] = x + long(nx) * y + plane * zm;
+  nb[5] = x + long(nx) * y + plane * zp;
+}
+
+// partition of all P phases with its derivatives: c_p, 1/t1_p and dc_p/dphi_q (0 where phi_p == 0,
+// as the forward kernels leave absent phases at zero weight)
+template <int P> struct PartD {
+  double cp[3], g[3], h[3][3];
+};
+template <int P>
+__device__ inline void part_deriv(double c, const double *ph, const DiffConsts<double> &dc,
+                                  PartD<P> &o) {
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    o.cp[i] = 0.0;
+    o.g[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) o.h[i][q] = 0.0;
+    if (ph[i] > 0.0) {
+      double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        t1 += ph[j] * dc.pk[i][j];
+        t2 += ph[j] * dc.pc[i][j];
+      }
+      o.cp[i] = (c - t2) / t1;
+      o.g[i] = 1.0 / t1;
+#pragma unroll
+      for (int q = 0; q < P; ++q) o.h[i][q] = -(dc.pc[i][q] + o.cp[i] * dc.pk[i][q]) / t1;
+    }
+  }
+}
+
+// Reverse of one sub-step: lam_out = dL/dc^s given lam_in = dL/dc^{s+1} and c^s; accumulates
+// dL/dvalue_phase into lam_pf and per-block partials of dL/dD_p into accD[block][p].
+template <int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void adj_diffuse_kernel(
+    const double *__restrict__ c, const double *__restrict__ pf, const double *__restrict__ lam_in,
+    double *__restrict__ lam_out, double *__restrict__ lam_pf, double *__restrict__ accD,
+    const DiffConsts<double> dc, int nx, int ny, int nz, long ncell) {
+  __shared__ double sh[P][256];
+  const long j = long(blockIdx.x) * 256 + threadIdx.x;
+  double dD[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) dD[p] = 0.0;
+  if (j < ncell) {
+    long nb[6];
+    nbrs(j, nx, ny, nz, NOFLUX, nb);
+    double phj[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) phj[p] = pf[j * P + p];
+    PartD<P> me;
+    part_deriv<P>(c[j], phj, dc, me);
+    const double lj = lam_in[j];
+    double lout = lj, lpf[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) lpf[p] = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const long n = nb[f];
+      double phn[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) phn[p] = pf[n * P + p];
+      PartD<P> o;
+      part_deriv<P>(c[n], phn, dc, o);
+      const double ln = lam_in[n];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const double prod = phn[p] * phj[p];
+        if (prod > 0.0) {
+          const double w = sqrt(prod);
+          const double dw = phn[p] / (2.0 * w);  // dw/dphi_p(j)
+          const double dcp = o.cp[p] - me.cp[p];
+          lout += dc.r * dc.D[p] * w * me.g[p] * (ln - lj);
+          lpf[p] += dc.r * dc.D[p] * dw * dcp * (lj - ln);
+#pragma unroll
+          for (int q = 0; q < P; ++q) lpf[q] += dc.r * dc.D[p] * w * me.h[p][q] * (ln - lj);
+          dD[p] += dc.r * lj * w * dcp;
+        }
+      }
+    }
+    lam_out[j] = lout;
+#pragma unroll
+    for (int p = 0; p < P; ++p) lam_pf[j * P + p] += lpf[p];
+  }
+  // deterministic per-block partial of dL/dD_p, added to this block's own slot
+#pragma unroll
+  for (int p = 0; p < P; ++p) sh[p][threadIdx.x] = dD[p];
+  __syncthreads();
+  for (int wdt = 128; wdt > 0; wdt >>= 1) {
+    if (int(threadIdx.x) < wdt)
+#pragma unroll
+      for (int p = 0; p < P; ++p) sh[p][threadIdx.x] += sh[p][threadIdx.x + wdt];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int p = 0; p < P; ++p) accD[blockIdx.x * 3 + p] += sh[p][0];
+}
+
+// Reverse of the phase-field sweep of one cell.  Inputs: the sweep's input state (ids, phi, pf,
+// c), lam_phi2/lam_pf2 = adjoints of its outputs.  Outputs: lam_phi1 (own-slot part), lam_pf1,
+// lam_c (+=), the neighbour-sum adjoint list of the cell (entry ids and values, count 0 when the
+// cell was copied) and per-block partials of dL/d(mobility scale), dL/d(sigma scale).
+template <int K, int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(kAdjBS) void adj_sweep_kernel(
+    const uint16_t *__restrict__ id1, const double *__restrict__ phi1,
+    const double *__restrict__ pf1, const double *__restrict__ c,
+    const double *__restrict__ lam_phi2, const double *__restrict__ lam_pf2,
+    double *__restrict__ lam_phi1, double *__restrict__ lam_pf1, double *__restrict__ lam_c,
+    uint32_t *__restrict__ v2_id, double *__restrict__ v2_lam, uint8_t *__restrict__ v2_n,
+    double *__restrict__ accMS, const PFConsts<double> cs, int nx, int ny, int nz, long ncell) {
+  __shared__ uint32_t sLg[kStencilMax][kAdjBS];
+  __shared__ double sV1[kStencilMax][kAdjBS], sV2[kStencilMax][kAdjBS];
+  __shared__ double sD[kStencilMax][kAdjBS], sL1[kStencilMax][kAdjBS], sL2[kStencilMax][kAdjBS];
+  __shared__ uint8_t sPerm[kStencilMax][kAdjBS], sLive[kStencilMax][kAdjBS];
+  __shared__ double red[2][kAdjBS];
+  const int t = threadIdx.x;
+  const long i = long(blockIdx.x) * kAdjBS + t;
+  double dMs = 0.0, dSs = 0.0;
+  const double cutoff = cs.cutoff, dt = cs.dt;
+  if (i < ncell) {
+    for (int e = 0; e < kStencilMax; ++e) {
+      sLg[e][t] = 0;
+      sV1[e][t] = sV2[e][t] = sD[e][t] = sL1[e][t] = sL2[e][t] = 0.0;
+      sPerm[e][t] = uint8_t(e);
+      sLive[e][t] = 0;
+    }
+    uint32_t g0[K];
+    load_ids<K>(id1, i, g0);
+    int n = 0;
+    for (int s = 0; s < K; ++s)
+      if (phi1[i * K + s] > cutoff) {
+        ++n;
+        sLg[s][t] = g0[s];
+        sV1[s][t] = phi1[i * K + s];
+      }
+    long nb[6];
+    nbrs(i, nx, ny, nz, NOFLUX, nb);
+    for (int f = 0; f < NF; ++f) {
+      uint32_t gn[K];
+      load_ids<K>(id1, nb[f], gn);
+      for (int s = 0; s < K; ++s) {
+        const uint32_t g = gn[s];
+        if (g == 0) break;
+        const double v = phi1[nb[f] * K + s];
+        int kk = 0;
+        for (; kk < kStencilMax; ++kk)
+          if (sLg[kk][t] == g) break;
+        if (kk < kStencilMax) {
+          sV2[kk][t] += v;
+        } else if (n < kStencilMax) {
+          sLg[n][t] = g;
+          sV2[n][t] += v;
+          ++n;
+        }
+      }
+    }
+    if (n == 1) {
+      // copied cell: slots and value_phase pass through unchanged
+      for (int s = 0; s < K; ++s) lam_phi1[i * K + s] += lam_phi2[i * K + s];
+      for (int p = 0; p < P; ++p) lam_pf1[i * P + p] += lam_pf2[i * P + p];
+      v2_n[i] = 0;
+    } else {
+      // local concentrations and driving forces (as the forward sweep)
+      double ph[P], lc[4] = {0, 0, 0, 0}, t1v[4] = {1, 1, 1, 1};
+      for (int j = 0; j < P; ++j) ph[j] = pf1[i * P + j];
+      for (int q = 1; q <= P; ++q) {
+        double t1 = 0.0, t2 = 0.0;
+        for (int j = 0; j < P; ++j) {
+          t1 += ph[j] * cs.pk[q][j + 1];
+          t2 += ph[j] * cs.pc[q][j + 1];
+        }
+        lc[q] = (c[i] - t2) / t1;
+        t1v[q] = t1;
+      }
+      auto dG = [&](int a, int b) -> double {
+        return (a != b && a >= 1 && b >= 1) ? cs.ent[a][b] * (lc[a] - cs.eqc[a][b]) / cs.slope[a][b]
+                                            : 0.0;
+      };
+      auto tab = [&](const double (&tb)[4][4], int a, int b) -> double {
+        return (a >= 1 && b >= 1 && a <= 3 && b <= 3) ? tb[a][b] : 0.0;
+      };
+      // pre-clamp updates; live[e]: the clamp and prune left the value differentiable
+      for (int a = 0; a < n; ++a) {
+        const int pa = code_phase(sLg[a][t]);
+        double acc = 0.0;
+        for (int b = 0; b < n; ++b) {
+          if (b == a) continue;
+          const int pb = code_phase(sLg[b][t]);
+          if (pa - pb == 2 || pb - pa == 2) continue;
+          double tt = (sV2[a][t] * sV1[b][t] - sV2[b][t] * sV1[a][t]) * cs.f1;
+          tt += (sV1[a][t] - sV1[b][t]) * cs.f2;
+          tt *= tab(cs.sig, pa, pb);
+          tt += cs.f3 * sqrt(sV1[a][t] * sV1[b][t]) * dG(pa, pb);
+          tt *= tab(cs.M, pa, pb);
+          acc += tt;
+        }
+        acc *= dt;
+        acc += sV1[a][t];
+        uint8_t live = 1;
+        if (acc > 1.0) {
+          acc = 1.0;
+          live = 0;
+        } else if (acc < cutoff) {
+          acc = 0.0;
+          live = 0;
+        }
+        sD[a][t] = acc;
+        sLive[a][t] = live;
+      }
+      // selection sort (first K passes) recording the permutation
+      for (int a = 0; a < K; ++a) {
+        int m = a;
+        double best = sD[a][t];
+        for (int b = a + 1; b < kStencilMax; ++b)
+          if (sD[b][t] > best) {
+            best = sD[b][t];
+            m = b;
+          }
+        if (m != a) {
+          sD[m][t] = sD[a][t];
+          sD[a][t] = best;
+          const uint8_t tp = sPerm[a][t];
+          sPerm[a][t] = sPerm[m][t];
+          sPerm[m][t] = tp;
+        }
+      }
+      // renormalisation adjoint: out(s) = D_s/S, pf'(p) = sum_{s of phase p} D_s / S
+      double S = 0.0, vp[4] = {0, 0, 0, 0};
+      bool kept[6];
+      int phs[6];
+      for (int s = 0; s < K; ++s) {
+        const uint32_t g = sLg[sPerm[s][t]][t];
+        kept[s] = sD[s][t] > cutoff;
+        phs[s] = code_phase(g);
+        if (kept[s]) {
+          S += sD[s][t];
+          vp[phs[s]] += sD[s][t];
+        }
+      }
+      double dot = 0.0, dotp = 0.0;
+      for (int s = 0; s < K; ++s)
+        if (kept[s]) dot += lam_phi2[i * K + s] * (sD[s][t] / S);
+      for (int p = 1; p <= P; ++p) dotp += lam_pf2[i * P + p - 1] * (vp[p] / S);
+      double lamD[kStencilMax];
+      for (int e = 0; e < kStencilMax; ++e) lamD[e] = 0.0;
+      for (int s = 0; s < K; ++s)
+        if (kept[s]) {
+          const int a = sPerm[s][t];
+          const double lp = phs[s] >= 1 ? lam_pf2[i * P + phs[s] - 1] : 0.0;
+          const double l = (lam_phi2[i * K + s] - dot + lp - dotp) / S;
+          lamD[a] = sLive[a][t] ? l : 0.0;
+        }
+      // pair-term adjoint
+      double lamLc[4] = {0, 0, 0, 0};
+      for (int a = 0; a < n; ++a) {
+        const double la = lamD[a];
+        if (la == 0.0) continue;
+        const int pa = code_phase(sLg[a][t]);
+        const double v1a = sV1[a][t], v2a = sV2[a][t];
+        sL1[a][t] += la;  // dD_a/dV1_a: the Euler identity term
+        for (int b = 0; b < n; ++b) {
+          if (b == a) continue;
+          const int pb = code_phase(sLg[b][t]);
+          if (pa - pb == 2 || pb - pa == 2) continue;
+          const double v1b = sV1[b][t], v2b = sV2[b][t];
+          const double sg = tab(cs.sig, pa, pb), mo = tab(cs.M, pa, pb), g = dG(pa, pb);
+          const double X = (v2a * v1b - v2b * v1a) * cs.f1 + (v1a - v1b) * cs.f2;
+          const double prod = v1a * v1b, sq = sqrt(prod);
+          const double k = la * dt * mo;
+          // t_ab = M [sigma X + f3 sqrt(v1a v1b) dG]
+          sL1[a][t] += k * (sg * (-v2b * cs.f1 + cs.f2) + (prod > 0 ? cs.f3 * g * v1b / (2 * sq) : 0));
+          sL1[b][t] += k * (sg * (v2a * cs.f1 - cs.f2) + (prod > 0 ? cs.f3 * g * v1a / (2 * sq) : 0));
+          sL2[a][t] += k * sg * v1b * cs.f1;
+          sL2[b][t] += -k * sg * v1a * cs.f1;
+          if (pa != pb && pa >= 1 && pb >= 1)
+            lamLc[pa] += k * cs.f3 * sq * cs.ent[pa][pb] / cs.slope[pa][pb];
+          dMs += la * dt * mo * (sg * X + cs.f3 * sq * g);
+          dSs += la * dt * mo * sg * X;
+        }
+      }
+      // own slot values entered as V1 of the entry at their slot position
+      for (int s = 0; s < K; ++s)
+        if (phi1[i * K + s] > cutoff) lam_phi1[i * K + s] += sL1[s][t];
+      // local concentration: lc_q = (c - t2_q)/t1_q of the input value_phase
+      double lcv = 0.0;
+      for (int q = 1; q <= P; ++q) {
+        if (lamLc[q] == 0.0) continue;
+        lcv += lamLc[q] / t1v[q];
+        for (int r = 0; r < P; ++r)
+          lam_pf1[i * P + r] +=
+              lamLc[q] * (-(cs.pc[q][r + 1] + lc[q] * cs.pk[q][r + 1]) / t1v[q]);
+      }
+      lam_c[i] += lcv;
+      for (int e = 0; e < kStencilMax; ++e) {
+        v2_id[i * kStencilMax + e] = e < n ? sLg[e][t] : 0u;
+        v2_lam[i * kStencilMax + e] = e < n ? sL2[e][t] : 0.0;
+      }
+      v2_n[i] = uint8_t(n);
+    }
+  }
+  red[0][t] = dMs;
+  red[1][t] = dSs;
+  __syncthreads();
+  for (int w = kAdjBS / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      red[0][t] += red[0][t + w];
+      red[1][t] += red[1][t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    accMS[blockIdx.x * 2] += red[0][0];
+    accMS[blockIdx.x * 2 + 1] += red[1][0];
+  }
+}
+
+// Neighbour-sum adjoints back to the slots that fed them: slot s of cell j (before the first empty
+// slot) was added to the entry with its id in every cell that has j as a stencil neighbour.
+template <int K, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void adj_gather_kernel(
+    const uint16_t *__restrict__ id1, const uint32_t *__restrict__ v2_id,
+    const double *__restrict__ v2_lam, const uint8_t *__restrict__ v2_n,
+    double *__restrict__ lam_phi1, int nx, int ny, int nz, long ncell) {
+  const long j = long(blockIdx.x) * 256 + threadIdx.x;
+  if (j >= ncell) return;
+  uint32_t g[K];
+  load_ids<K>(id1, j, g);
+  long nb[6];
+  nbrs(j, nx, ny, nz, NOFLUX, nb);
+  for (int s = 0; s < K; ++s) {
+    if (g[s] == 0) break;
+    double acc = 0.0;
+    for (int f = 0; f < NF; ++f) {
+      const long i = nb[f];
+      const int n = v2_n[i];
+      for (int e = 0; e < n; ++e)
+        if (v2_id[i * kStencilMax + e] == g[s]) {
+          acc += v2_lam[i * kStencilMax + e];
+          break;
+        }
+    }
+    lam_phi1[j * K + s] += acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void adj_seed_fraction_kernel(double *__restrict__ lam_pf,
+                                                                 int P, double v, long ncell) {
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i < ncell) lam_pf[i * P] += v;
+}
+
+__global__ __launch_bounds__(256) void adj_seed_conc_kernel(const double *__restrict__ c,
+                                                            const double *__restrict__ target,
+                                                            double w, double *__restrict__ lam,
+                                                            double *__restrict__ part,
+                                                            long ncell) {
+  __shared__ double sh[256];
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  double l = 0.0;
+  if (i < ncell) {
+    const double d = c[i] - target[i];
+    lam[i] += 2.0 * w * d;
+    l = w * d * d;
+  }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int wdt = 128; wdt > 0; wdt >>= 1) {
+    if (int(threadIdx.x) < wdt) sh[threadIdx.x] += sh[threadIdx.x + wdt];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+}  // namespace pfamd
+
+// ================================================================================================
+// host orchestration (the handle's buffers through the internal view defined in pf_api.hip)
+
+using namespace pfamd;
+
+namespace {
+
+struct DevState {  // one FP64 state on the device
+  uint16_t *id = nullptr;
+  double *phi = nullptr, *pf = nullptr, *c = nullptr;
+};
+
+hipError_t alloc_state(DevState &s, long N, int KI, int K, int P) {
+  hipError_t e = hipMalloc(&s.id, size_t(N) * KI * 2);
+  if (e == hipSuccess) e = hipMalloc(&s.phi, size_t(N) * K * 8);
+  if (e == hipSuccess) e = hipMalloc(&s.pf, size_t(N) * P * 8);
+  if (e == hipSuccess) e = hipMalloc(&s.c, size_t(N) * 8);
+  return e;
+}
+void free_state(DevState &s) {
+  if (s.id) hipFree(s.id);
+  if (s.phi) hipFree(s.phi);
+  if (s.pf) hipFree(s.pf);
+  if (s.c) hipFree(s.c);
+  s = DevState{};
+}
+
+template <int K, int P, int NF, bool NOF> struct Adj {
+  static int run(pf_handle *h, const pf_adjoint_spec *sp, pf_adjoint_result *out) {
+    HandleView v = handle_view(h);
+    const long N = v.N;
+    const int KI = v.KI;
+    hipStream_t st = v.stream;
+    const pf_params &p = *v.params;
+    const int nsteps = sp->n_steps, CK = sp->checkpoint_every, nsub = p.n_sub[0];
+    const double factor = p.dt_d[0] / (p.dx * p.dx);
+    const DiffConsts<double> dc = handle_diff_consts(h, 0, factor);
+    const PFConsts<double> cs = handle_pf_consts(h);
+    const int nseg = (nsteps + CK - 1) / CK;
+    const unsigned gb = unsigned((N + 255) / 256), gs = unsigned((N + kAdjBS - 1) / kAdjBS);
+    // device buffers
+    std::vector<DevState> ckpt(nseg), seg(std::min(CK, nsteps));
+    DevState work;
+    double *csub = nullptr, *lam_c = nullptr, *lam_c2 = nullptr, *lam_phi = nullptr,
+           *lam_phi2 = nullptr, *lam_pf = nullptr, *lam_pf2 = nullptr, *v2_lam = nullptr,
+           *accD = nullptr, *accMS = nullptr, *part = nullptr, *target = nullptr;
+    uint32_t *v2_id = nullptr;
+    uint8_t *v2_n = nullptr;
+    std::string err;
+    auto cleanup = [&]() {
+      for (auto &s : ckpt) free_state(s);
+      for (auto &s : seg) free_state(s);
+      free_state(work);
+      for (void *q : {(void *)csub, (void *)lam_c, (void *)lam_c2, (void *)lam_phi,
+                      (void *)lam_phi2, (void *)lam_pf, (void *)lam_pf2, (void *)v2_lam,
+                      (void *)accD, (void *)accMS, (void *)part, (void *)target, (void *)v2_id,
+                      (void *)v2_n})
+        if (q) hipFree(q);
+    };
+    hipError_t e = hipSuccess;
+    for (auto &s : ckpt)
+      if (e == hipSuccess) e = alloc_state(s, N, KI, K, P);
+    for (auto &s : seg)
+      if (e == hipSuccess) e = alloc_state(s, N, KI, K, P);
+    if (e == hipSuccess) e = alloc_state(work, N, KI, K, P);
+    if (e == hipSuccess) e = hipMalloc(&csub, size_t(N) * 8 * (nsub + 1));
+    for (double **q : {&lam_c, &lam_c2, &lam_pf, &lam_pf2})
+      if (e == hipSuccess) e = hipMalloc(q, size_t(N) * 8 * (q == &lam_pf || q == &lam_pf2 ? P : 1));
+    if (e == hipSuccess) e = hipMalloc(&lam_phi, size_t(N) * K * 8);
+    if (e == hipSuccess) e = hipMalloc(&lam_phi2, size_t(N) * K * 8);
+    if (e == hipSuccess) e = hipMalloc(&v2_lam, size_t(N) * kStencilMax * 8);
+    if (e == hipSuccess) e = hipMalloc(&v2_id, size_t(N) * kStencilMax * 4);
+    if (e == hipSuccess) e = hipMalloc(&v2_n, size_t(N));
+    if (e == hipSuccess) e = hipMalloc(&accD, size_t(gb) * 3 * 8);
+    if (e == hipSuccess) e = hipMalloc(&accMS, size_t(gs) * 2 * 8);
+    if (e == hipSuccess) e = hipMalloc(&part, size_t(gb) * 8);
+    if (e == hipSuccess && sp->target_conc) e = hipMalloc(&target, size_t(N) * 8);
+    if (e != hipSuccess) {
+      cleanup();
+      return handle_fail(h, PF_ERR_OOM, "adjoint buffers: " + std::string(hipGetErrorString(e)));
+    }
+    auto copy_state = [&](DevState &dst, const DevState &src) {
+      hipMemcpyAsync(dst.id, src.id, size_t(N) * KI * 2, hipMemcpyDeviceToDevice, st);
+      hipMemcpyAsync(dst.phi, src.phi, size_t(N) * K * 8, hipMemcpyDeviceToDevice, st);
+      hipMemcpyAsync(dst.pf, src.pf, size_t(N) * P * 8, hipMemcpyDeviceToDevice, st);
+      hipMemcpyAsync(dst.c, src.c, size_t(N) * 8, hipMemcpyDeviceToDevice, st);
+    };
+    // the handle's current state <-> DevState
+    auto from_handle = [&](DevState &dst) {
+      HandleView w = handle_view(h);
+      DevState src{w.id, static_cast<double *>(w.phi), static_cast<double *>(w.pf),
+                   static_cast<double *>(w.c0)};
+      copy_state(dst, src);
+    };
+    auto to_handle = [&](const DevState &src) {
+      HandleView w = handle_view(h);
+      DevState dst{w.id, static_cast<double *>(w.phi), static_cast<double *>(w.pf),
+                   static_cast<double *>(w.c0)};
+      copy_state(dst, src);
+      handle_state_changed(h);
+    };
+    if (target)
+      hipMemcpyAsync(target, sp->target_conc, size_t(N) * 8, hipMemcpyHostToDevice, st);
+    // ---- forward with checkpoints ----
+    auto t0 = std::chrono::steady_clock::now();
+    double loss = 0.0;
+    std::vector<double> frac(std::max(sp->n_samples, 0), 0.0);
+    auto sample_index = [&](int step) -> int {
+      for (int q = 0; q < sp->n_samples; ++q)
+        if (sp->sample_steps[q] == step) return q;
+      return -1;
+    };
+    int rc = PF_OK;
+    for (int k = 0; k < nsteps && rc == PF_OK; ++k) {
+      if (k % CK == 0) from_handle(ckpt[k / CK]);
+      rc = pf_step(h, 1);
+      const int q = sample_index(k + 1);
+      if (rc == PF_OK && q >= 0) {
+        pf_stats s;
+        rc = pf_reduce(h, &s);
+        frac[q] = s.phase_frac[0];
+        const double d = frac[q] - sp->target_fraction[q];
+        loss += d * d;
+      }
+    }
+    DevState final_state;
+    if (rc == PF_OK) {
+      e = alloc_state(final_state, N, KI, K, P);
+      if (e == hipSuccess) from_handle(final_state);
+    }
+    if (rc != PF_OK || e != hipSuccess) {
+      free_state(final_state);
+      cleanup();
+      return rc != PF_OK ? rc : handle_fail(h, PF_ERR_OOM, "adjoint final state");
+    }
+    // loss concentration term and its seed
+    hipMemsetAsync(lam_c, 0, size_t(N) * 8, st);
+    hipMemsetAsync(lam_phi, 0, size_t(N) * K * 8, st);
+    hipMemsetAsync(lam_pf, 0, size_t(N) * P * 8, st);
+    hipMemsetAsync(accD, 0, size_t(gb) * 3 * 8, st);
+    hipMemsetAsync(accMS, 0, size_t(gs) * 2 * 8, st);
+    if (target && sp->weight_conc != 0.0) {
+      hipLaunchKernelGGL(adj_seed_conc_kernel, dim3(gb), dim3(256), 0, st, final_state.c, target,
+                         sp->weight_conc, lam_c, part, N);
+      std::vector<double> hp(gb);
+      hipMemcpyAsync(hp.data(), part, size_t(gb) * 8, hipMemcpyDeviceToHost, st);
+      hipStreamSynchronize(st);
+      for (double x : hp) loss += x;
+    }
+    hipStreamSynchronize(st);
+    auto t1 = std::chrono::steady_clock::now();
+    // ---- reverse ----
+    for (int sg = nseg - 1; sg >= 0 && rc == PF_OK; --sg) {
+      const int k0 = sg * CK, k1 = std::min(nsteps, k0 + CK);
+      // recompute the segment, keeping each step's input state
+      to_handle(ckpt[sg]);
+      for (int k = k0; k < k1 && rc == PF_OK; ++k) {
+        from_handle(seg[k - k0]);
+        if (k + 1 < k1) rc = pf_step(h, 1);
+      }
+      for (int k = k1 - 1; k >= k0 && rc == PF_OK; --k) {
+        // loss seed on value_phase after step k
+        const int q = sample_index(k + 1);
+        if (q >= 0) {
+          const double g = 2.0 * (frac[q] - sp->target_fraction[q]) / double(N);
+          hipLaunchKernelGGL(adj_seed_fraction_kernel, dim3(gb), dim3(256), 0, st, lam_pf, P, g,
+                             N);
+        }
+        // recompute the step: sweep, then the sub-steps storing every concentration
+        to_handle(seg[k - k0]);
+        rc = pf_phase_field_sweep(h);
+        if (rc != PF_OK) break;
+        HandleView w = handle_view(h);
+        const double *pf_new = static_cast<const double *>(w.pf);
+        hipMemcpyAsync(csub, w.c0, size_t(N) * 8, hipMemcpyDeviceToDevice, st);
+        for (int s = 0; s < nsub && rc == PF_OK; ++s) {
+          rc = pf_diffuse_substeps(h, 0, 1, p.dt_d[0]);
+          HandleView w2 = handle_view(h);
+          hipMemcpyAsync(csub + size_t(N) * (s + 1), w2.c0, size_t(N) * 8,
+                         hipMemcpyDeviceToDevice, st);
+        }
+        if (rc != PF_OK) break;
+        // reverse sub-steps: lam_pf collects the value_phase adjoint of the sweep output
+        for (int s = nsub - 1; s >= 0; --s) {
+          hipLaunchKernelGGL((adj_diffuse_kernel<P, NF, NOF>), dim3(gb), dim3(256), 0, st,
+                             csub + size_t(N) * s, pf_new, lam_c, lam_c2, lam_pf, accD, dc,
+                             v.nx, v.ny, v.nz, N);
+          std::swap(lam_c, lam_c2);
+        }
+        // reverse sweep: sweep input is seg[k - k0]
+        const DevState &in = seg[k - k0];
+        hipMemsetAsync(lam_phi2, 0, size_t(N) * K * 8, st);
+        hipMemsetAsync(lam_pf2, 0, size_t(N) * P * 8, st);
+        hipLaunchKernelGGL((adj_sweep_kernel<K, P, NF, NOF>), dim3(gs), dim3(kAdjBS), 0, st,
+                           in.id, in.phi, in.pf, in.c, lam_phi, lam_pf, lam_phi2, lam_pf2, lam_c,
+                           v2_id, v2_lam, v2_n, accMS, cs, v.nx, v.ny, v.nz, N);
+        hipLaunchKernelGGL((adj_gather_kernel<K, NF, NOF>), dim3(gb), dim3(256), 0, st, in.id,
+                           v2_id, v2_lam, v2_n, lam_phi2, v.nx, v.ny, v.nz, N);
+        std::swap(lam_phi, lam_phi2);
+        std::swap(lam_pf, lam_pf2);
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+          rc = handle_fail(h, PF_ERR_HIP, std::string("adjoint: ") + hipGetErrorString(e));
+          break;
+        }
+      }
+    }
+    if (rc == PF_OK) {
+      std::vector<double> hD(size_t(gb) * 3), hMS(size_t(gs) * 2);
+      hipMemcpyAsync(hD.data(), accD, hD.size() * 8, hipMemcpyDeviceToHost, st);
+      hipMemcpyAsync(hMS.data(), accMS, hMS.size() * 8, hipMemcpyDeviceToHost, st);
+      if (out->d_conc0)
+        hipMemcpyAsync(out->d_conc0, lam_c, size_t(N) * 8, hipMemcpyDeviceToHost, st);
+      e = hipStreamSynchronize(st);
+      if (e != hipSuccess) rc = handle_fail(h, PF_ERR_HIP, hipGetErrorString(e));
+      double dD = 0.0, dM = 0.0, dS = 0.0;
+      for (unsigned b = 0; b < gb; ++b) dD += hD[size_t(b) * 3];
+      for (unsigned b = 0; b < gs; ++b) {
+        dM += hMS[size_t(b) * 2];
+        dS += hMS[size_t(b) * 2 + 1];
+      }
+      out->loss = loss;
+      out->d_mobility = dM;
+      out->d_sigma = dS;
+      out->d_diff_gamma = dD;
+      for (int q = 0; q < sp->n_samples && q < 64; ++q) out->fraction[q] = frac[q];
+      auto t2 = std::chrono::steady_clock::now();
+      out->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+      out->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    }
+    // leave the final forward state on the handle
+    to_handle(final_state);
+    hipStreamSynchronize(st);
+    free_state(final_state);
+    cleanup();
+    return rc;
+  }
+};
+
+template <int K, int P> int run_nf(pf_handle *h, const pf_adjoint_spec *sp, pf_adjoint_result *o) {
+  HandleView v = handle_view(h);
+  if (v.NF == 4) return Adj<K, P, 4, false>::run(h, sp, o);
+  if (v.noflux) return Adj<K, P, 6, true>::run(h, sp, o);
+  return Adj<K, P, 6, false>::run(h, sp, o);
+}
+
+}  // namespace
 
 extern "C" int pf_adjoint_run(pf_handle *h, const pf_adjoint_spec *spec, pf_adjoint_result *out) {
-  (void)h;
-  (void)spec;
-  (void)out;
-  return PF_ERR_INVALID;
+  if (!h || !spec || !out) return PF_ERR_INVALID;
+  HandleView v = handle_view(h);
+  if (v.mixed) return handle_fail(h, PF_ERR_INVALID, "the adjoint runs on FP64 domains");
+  if (v.nsolute != 1) return handle_fail(h, PF_ERR_INVALID, "the adjoint covers the carbon-only step");
+  if (!v.have_state || !v.params) return handle_fail(h, PF_ERR_STATE, "state or parameters missing");
+  if (spec->n_steps < 1 || spec->checkpoint_every < 1 || spec->n_samples < 0 ||
+      spec->n_samples > 64)
+    return handle_fail(h, PF_ERR_INVALID, "bad adjoint spec");
+  hipSetDevice(v.device);
+  if (v.K != 4 && v.K != 6) return handle_fail(h, PF_ERR_INVALID, "depth must be 4 or 6");
+  if (v.K == 4 && v.P == 2) return run_nf<4, 2>(h, spec, out);
+  if (v.K == 4 && v.P == 3) return run_nf<4, 3>(h, spec, out);
+  if (v.K == 6 && v.P == 2) return run_nf<6, 2>(h, spec, out);
+  return run_nf<6, 3>(h, spec, out);
 }

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/phasefield_amd.h"
+#include "pf_handle_access.h"
 #include "pf_kernels.h"
 
 using namespace pfamd;
@@ -1066,3 +1067,34 @@ void GPU_Cleanup(void) {
 }
 
 }  // extern "C"
+
+// ---- internal view for the adjoint translation unit ---------------------------------------------
+HandleView handle_view(pf_handle *h) {
+  HandleView v;
+  v.N = h->N;
+  v.nx = h->d.nx;
+  v.ny = h->d.ny;
+  v.nz = h->d.nz;
+  v.K = h->d.depth;
+  v.KI = h->KI;
+  v.P = h->d.nphase;
+  v.NF = h->NF;
+  v.nsolute = h->d.nsolute;
+  v.device = h->d.device;
+  v.noflux = h->d.bc_z == PF_BC_NOFLUX;
+  v.mixed = is_mixed(h);
+  v.have_state = h->have_state;
+  v.params = h->have_params ? &h->p : nullptr;
+  v.stream = h->stream;
+  v.id = h->id[h->cur];
+  v.phi = h->phi[h->cur];
+  v.pf = h->pfrac;
+  v.c0 = h->conc[0][h->ccur[0]];
+  return v;
+}
+int handle_fail(pf_handle *h, int code, const std::string &msg) { return fail(h, code, msg); }
+void handle_state_changed(pf_handle *h) { h->flag_valid = false; }
+PFConsts<double> handle_pf_consts(const pf_handle *h) { return make_pf_consts<double>(h); }
+DiffConsts<double> handle_diff_consts(const pf_handle *h, int solute, double factor) {
+  return make_diff_consts<double>(h->p, solute, factor);
+}
