@@ -113,6 +113,99 @@ def cpu_baseline(workload: str, tmp: str):
                       f"(8 grains, 16 nuclei), FP64, {dt:.1f} s"}
 
 
+def adjoint_section(tmp: str, steps: int | None, fd: list):
+    """Discrete adjoint on the 128^3 Fe-C adjoint workload: targets from a forward run with
+    mobility x1.1 and D_gamma x0.9, loss = sum over 20 sample times of (f_gamma - target)^2
+    + 1e-3 ||c_final - c_target||^2, checkpoints every 20 steps.  Reports the reverse sweep's cost
+    against a plain forward run of the same steps, and central finite differences (relative step
+    1e-4) of the four gradients."""
+    import numpy as np
+
+    from phasefield_amd import Model, configs, params_from_dict, params_to_dict
+    from phasefield_amd.adjoint import AdjointProblem
+
+    wl = "fec_3d_128_adjoint"
+    nsteps = steps or configs.WORKLOADS[wl]["steps"]
+    ck = max(1, nsteps // 20)
+    m = Model(configs.write_inputs(os.path.join(tmp, "adj"), wl))
+    p0 = m.update_parameter()
+    log(f"adjoint: setup {wl}, {nsteps} steps, checkpoints every {ck}")
+    g = setup_fp64(m, p0)
+    s0 = g.download()
+    st = (s0["index"], s0["value"], s0["phase"], s0["conc"].copy())
+    N = g.N
+    samples = [ck * (q + 1) for q in range(nsteps // ck)][:20]
+    base = params_to_dict(p0)
+
+    def forward_loss(theta, c0, targets=None, c_target=None, wc=1e-3):
+        prob.domain.set_params(params_from_dict(prob.params_at(theta)))
+        g.upload(st[0], st[1], st[2], c0)
+        fr, loss = [], 0.0
+        for k in range(1, nsteps + 1):
+            g.step(1)
+            if k in samples:
+                fr.append(g.reduce()["phase_frac"][0])
+        c = g.download()["conc"]
+        if targets is not None:
+            loss = sum((a - b) ** 2 for a, b in zip(fr, targets)) + wc * float(np.sum((c - c_target) ** 2))
+        return loss, fr, c
+
+    prob = AdjointProblem(g, base, st[0], st[1], st[2], nsteps, samples, [0.0] * len(samples), 1e-3,
+                          None, ck)
+    th0 = prob.theta0().numpy()
+    # targets: mobility x1.1, D_gamma x0.9
+    _, f_t, c_t = forward_loss(th0 * np.array([1.1, 1.0, 0.9]), st[3])
+    prob.target_fraction = f_t
+    prob.target_conc = c_t
+    # plain forward of the same steps (no sampling), timed
+    g.set_params(params_from_dict(prob.params_at(th0)))
+    g.upload(*st)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.step(nsteps)
+    g.synchronize()
+    fwd_s = time.perf_counter() - t0
+    r = prob.run(th0, st[3])
+    out = {"workload": f"{wl}: 128^3 Fe-C, 32 grains + 40 nuclei, {nsteps} steps, "
+                       f"{p0.n_sub[0]} sub-steps/step, checkpoints every {ck}, 20 samples",
+           "loss": r["loss"], "forward_ms_per_step": round(fwd_s * 1e3 / nsteps, 4),
+           "adjoint_forward_ms_per_step": round(r["forward_ms"] / nsteps, 4),
+           "adjoint_reverse_ms_per_step": round(r["reverse_ms"] / nsteps, 4),
+           "reverse_over_forward": round(r["reverse_ms"] / (fwd_s * 1e3), 3),
+           "forward_plus_reverse_over_forward": round((r["forward_ms"] + r["reverse_ms"]) /
+                                                      (fwd_s * 1e3), 3),
+           "grad": {"mobility_scale": r["grad_theta"][0], "sigma_scale": r["grad_theta"][1],
+                    "D_gamma": r["grad_theta"][2],
+                    "c0_norm": float(np.linalg.norm(r["d_conc0"]))}}
+    log(f"adjoint: forward {out['forward_ms_per_step']} ms/step, reverse "
+        f"{out['adjoint_reverse_ms_per_step']} ms/step ({out['reverse_over_forward']}x)")
+    if fd:
+        rng = np.random.default_rng(2024)
+        grads = list(r["grad_theta"])
+        direction = st[3] * rng.standard_normal(N)
+        grads.append(float(r["d_conc0"] @ direction))
+        out["fd_check"] = {"params": ["mobility_scale", "sigma_scale", "D_gamma",
+                                      "c0 . (c0 * N(0,1))"], "adjoint": grads, "by_rel_step": {}}
+        for h in fd:
+            rows = []
+            for k in range(4):
+                if k < 3:
+                    e = np.zeros(3)
+                    e[k] = h * th0[k]
+                    lp = forward_loss(th0 + e, st[3], f_t, c_t)[0]
+                    lm = forward_loss(th0 - e, st[3], f_t, c_t)[0]
+                    fdv = (lp - lm) / (2 * e[k])
+                else:
+                    lp = forward_loss(th0, st[3] + h * direction, f_t, c_t)[0]
+                    lm = forward_loss(th0, st[3] - h * direction, f_t, c_t)[0]
+                    fdv = (lp - lm) / (2 * h)
+                rows.append({"fd": fdv, "rel_err": abs(grads[k] - fdv) / max(abs(fdv), 1e-300)})
+            out["fd_check"]["by_rel_step"][f"{h:g}"] = rows
+            log(f"adjoint: FD h={h:g} rel errors {[round(x['rel_err'], 6) for x in rows]}")
+    g.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +214,12 @@ def main() -> int:
     ap.add_argument("--workload", default="fecmn_3d_512")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=0)
+    ap.add_argument("--adjoint-fd", default="", metavar="H1,H2,..",
+                    help="central finite differences of the adjoint gradients at these relative "
+                         "steps (8 forward runs each), e.g. 1e-4,1e-6")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -143,6 +242,11 @@ def main() -> int:
     from phasefield_amd import Domain, Model, PF_MIXED, configs
 
     tmp = tempfile.mkdtemp(prefix="pfbench_")
+    fd_steps = [float(x) for x in args.adjoint_fd.split(",") if x.strip()]
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_section(tmp, args.adjoint_steps or None,
+                                                     fd_steps)}), flush=True)
+        return 0
     wl = configs.WORKLOADS[args.workload]
     m_mixed = Model(configs.write_inputs(os.path.join(tmp, "mixed"), args.workload,
                                          precision="mixed"))
@@ -220,6 +324,10 @@ def main() -> int:
             "mass_drift_rel_fp64": [abs(a - b) / b for a, b in zip(s64["mass"][: d.nsolute],
                                                                    st0["mass"][: d.nsolute])],
         }
+    mixed.close()
+    g64.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_section(tmp, args.adjoint_steps or None, fd_steps)
     if not args.no_cpu:
         log("cpu baseline (oracle, OpenMP)")
         out["cpu_baseline"] = cpu_baseline(args.workload, tmp)

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -81,71 +82,97 @@ __device__ inline void part_deriv(double c, const double *ph, const DiffConsts<d
   }
 }
 
+// sum of one double over the workgroup (256 lanes = 4 wavefronts), added to acc[blockIdx.x]
+__device__ inline void block_accumulate(double v, double *__restrict__ acc) {
+  __shared__ double part[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) part[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) acc[blockIdx.x] += (part[0] + part[1]) + (part[2] + part[3]);
+}
+
 // Reverse of one sub-step: lam_out = dL/dc^s given lam_in = dL/dc^{s+1} and c^s; accumulates
-// dL/dvalue_phase into lam_pf and per-block partials of dL/dD_p into accD[block][p].
+// dL/dvalue_phase into lam_pf and a per-workgroup partial of dL/dD(phase 1) into accD.
+// Bulk cells (flag p: the cell and its stencil hold exactly e_p, and the partition of phase p is
+// the identity) have unit weights and dc_p/dc = 1, so lam_out = lam + r D_p sum_faces (lam_nb -
+// lam).  The value_phase adjoint of a cell whose value_phase is exactly a unit vector is never
+// read (the renormalisation of such an output has zero derivative, a copied cell passes it on to
+// the same unit vector, and the initial value_phase is not a gradient output), so those cells
+// skip it.
 template <int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void adj_diffuse_kernel(
-    const double *__restrict__ c, const double *__restrict__ pf, const double *__restrict__ lam_in,
-    double *__restrict__ lam_out, double *__restrict__ lam_pf, double *__restrict__ accD,
-    const DiffConsts<double> dc, int nx, int ny, int nz, long ncell) {
-  __shared__ double sh[P][256];
+    const double *__restrict__ c, const double *__restrict__ pf, const uint8_t *__restrict__ flag,
+    const double *__restrict__ lam_in, double *__restrict__ lam_out, double *__restrict__ lam_pf,
+    double *__restrict__ accD, const DiffConsts<double> dc, int nx, int ny, int nz, long ncell) {
   const long j = long(blockIdx.x) * 256 + threadIdx.x;
-  double dD[P];
-#pragma unroll
-  for (int p = 0; p < P; ++p) dD[p] = 0.0;
+  double dD0 = 0.0;
   if (j < ncell) {
     long nb[6];
     nbrs(j, nx, ny, nz, NOFLUX, nb);
-    double phj[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) phj[p] = pf[j * P + p];
-    PartD<P> me;
-    part_deriv<P>(c[j], phj, dc, me);
     const double lj = lam_in[j];
-    double lout = lj, lpf[P];
+    const int fp = flag[j];
+    if (fp != 0) {
+      double sl = 0.0;
 #pragma unroll
-    for (int p = 0; p < P; ++p) lpf[p] = 0.0;
+      for (int f = 0; f < NF; ++f) sl += lam_in[nb[f]] - lj;
+      lam_out[j] = lj + dc.r * dc.D[fp - 1] * sl;
+      if (fp == 1) {
+        const double cj = c[j];
+        double sc = 0.0;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      const long n = nb[f];
-      double phn[P];
-#pragma unroll
-      for (int p = 0; p < P; ++p) phn[p] = pf[n * P + p];
-      PartD<P> o;
-      part_deriv<P>(c[n], phn, dc, o);
-      const double ln = lam_in[n];
+        for (int f = 0; f < NF; ++f) sc += c[nb[f]] - cj;
+        dD0 = dc.r * lj * sc;
+      }
+    } else {
+      double phj[P];
+      int ones = 0, zeros = 0;
 #pragma unroll
       for (int p = 0; p < P; ++p) {
-        const double prod = phn[p] * phj[p];
-        if (prod > 0.0) {
-          const double w = sqrt(prod);
-          const double dw = phn[p] / (2.0 * w);  // dw/dphi_p(j)
-          const double dcp = o.cp[p] - me.cp[p];
-          lout += dc.r * dc.D[p] * w * me.g[p] * (ln - lj);
-          lpf[p] += dc.r * dc.D[p] * dw * dcp * (lj - ln);
+        phj[p] = pf[j * P + p];
+        ones += phj[p] == 1.0;
+        zeros += phj[p] == 0.0;
+      }
+      const bool unit = ones == 1 && zeros == P - 1;
+      PartD<P> me;
+      part_deriv<P>(c[j], phj, dc, me);
+      double lout = lj, lpf[P];
 #pragma unroll
-          for (int q = 0; q < P; ++q) lpf[q] += dc.r * dc.D[p] * w * me.h[p][q] * (ln - lj);
-          dD[p] += dc.r * lj * w * dcp;
+      for (int p = 0; p < P; ++p) lpf[p] = 0.0;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const long n = nb[f];
+        double phn[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) phn[p] = pf[n * P + p];
+        PartD<P> o;
+        part_deriv<P>(c[n], phn, dc, o);
+        const double ln = lam_in[n];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const double prod = phn[p] * phj[p];
+          if (prod > 0.0) {
+            const double w = sqrt(prod);
+            const double dcp = o.cp[p] - me.cp[p];
+            lout += dc.r * dc.D[p] * w * me.g[p] * (ln - lj);
+            if (!unit) {
+              const double dw = phn[p] / (2.0 * w);  // dw/dphi_p(j)
+              lpf[p] += dc.r * dc.D[p] * dw * dcp * (lj - ln);
+#pragma unroll
+              for (int q = 0; q < P; ++q) lpf[q] += dc.r * dc.D[p] * w * me.h[p][q] * (ln - lj);
+            }
+            if (p == 0) dD0 += dc.r * lj * w * dcp;
+          }
         }
       }
+      lam_out[j] = lout;
+      if (!unit)
+#pragma unroll
+        for (int p = 0; p < P; ++p) lam_pf[j * P + p] += lpf[p];
     }
-    lam_out[j] = lout;
-#pragma unroll
-    for (int p = 0; p < P; ++p) lam_pf[j * P + p] += lpf[p];
   }
-  // deterministic per-block partial of dL/dD_p, added to this block's own slot
-#pragma unroll
-  for (int p = 0; p < P; ++p) sh[p][threadIdx.x] = dD[p];
-  __syncthreads();
-  for (int wdt = 128; wdt > 0; wdt >>= 1) {
-    if (int(threadIdx.x) < wdt)
-#pragma unroll
-      for (int p = 0; p < P; ++p) sh[p][threadIdx.x] += sh[p][threadIdx.x + wdt];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int p = 0; p < P; ++p) accD[blockIdx.x * 3 + p] += sh[p][0];
+  block_accumulate(dD0, accD);
 }
 
 // Reverse of the phase-field sweep of one cell.  Inputs: the sweep's input state (ids, phi, pf,
@@ -437,109 +464,129 @@ struct DevState {  // one FP64 state on the device
   double *phi = nullptr, *pf = nullptr, *c = nullptr;
 };
 
-hipError_t alloc_state(DevState &s, long N, int KI, int K, int P) {
-  hipError_t e = hipMalloc(&s.id, size_t(N) * KI * 2);
-  if (e == hipSuccess) e = hipMalloc(&s.phi, size_t(N) * K * 8);
-  if (e == hipSuccess) e = hipMalloc(&s.pf, size_t(N) * P * 8);
-  if (e == hipSuccess) e = hipMalloc(&s.c, size_t(N) * 8);
-  return e;
-}
-void free_state(DevState &s) {
-  if (s.id) hipFree(s.id);
-  if (s.phi) hipFree(s.phi);
-  if (s.pf) hipFree(s.pf);
-  if (s.c) hipFree(s.c);
-  s = DevState{};
-}
+// the device allocations of one adjoint run, released together
+struct Arena {
+  std::vector<void *> ptrs;
+  hipError_t err = hipSuccess;
+  template <class T> T *get(size_t count) {
+    void *q = nullptr;
+    if (err == hipSuccess) err = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+    if (err != hipSuccess) return nullptr;
+    ptrs.push_back(q);
+    return static_cast<T *>(q);
+  }
+  DevState state(long N, int KI, int K, int P) {
+    DevState d;
+    d.id = get<uint16_t>(size_t(N) * KI);
+    d.phi = get<double>(size_t(N) * K);
+    d.pf = get<double>(size_t(N) * P);
+    d.c = get<double>(size_t(N));
+    return d;
+  }
+  ~Arena() {
+    for (void *q : ptrs) hipFree(q);
+  }
+};
+
+// One step recomputed in the reverse sweep: its input state, value_phase after its sweep with the
+// bulk flags of that value_phase, and the carbon field before every sub-step (csub + N*s,
+// s = 0..n_sub; s = n_sub is the step's output).
+struct StepData {
+  DevState in;
+  double *pf_new = nullptr;
+  uint8_t *flag = nullptr;
+  double *csub = nullptr;
+};
 
 template <int K, int P, int NF, bool NOF> struct Adj {
   static int run(pf_handle *h, const pf_adjoint_spec *sp, pf_adjoint_result *out) {
-    HandleView v = handle_view(h);
+    const HandleView v = handle_view(h);
     const long N = v.N;
     const int KI = v.KI;
-    hipStream_t st = v.stream;
+    const hipStream_t st = v.stream;
     const pf_params &p = *v.params;
-    const int nsteps = sp->n_steps, CK = sp->checkpoint_every, nsub = p.n_sub[0];
-    const double factor = p.dt_d[0] / (p.dx * p.dx);
-    const DiffConsts<double> dc = handle_diff_consts(h, 0, factor);
-    const PFConsts<double> cs = handle_pf_consts(h);
+    const int nsteps = sp->n_steps, CK = std::min(sp->checkpoint_every, sp->n_steps);
+    const int nsub = std::max(p.n_sub[0], 0);
     const int nseg = (nsteps + CK - 1) / CK;
+    const DiffConsts<double> dc = handle_diff_consts(h, 0, p.dt_d[0] / (p.dx * p.dx));
+    const PFConsts<double> cs = handle_pf_consts(h);
     const unsigned gb = unsigned((N + 255) / 256), gs = unsigned((N + kAdjBS - 1) / kAdjBS);
-    // device buffers
-    std::vector<DevState> ckpt(nseg), seg(std::min(CK, nsteps));
-    DevState work;
-    double *csub = nullptr, *lam_c = nullptr, *lam_c2 = nullptr, *lam_phi = nullptr,
-           *lam_phi2 = nullptr, *lam_pf = nullptr, *lam_pf2 = nullptr, *v2_lam = nullptr,
-           *accD = nullptr, *accMS = nullptr, *part = nullptr, *target = nullptr;
-    uint32_t *v2_id = nullptr;
-    uint8_t *v2_n = nullptr;
-    std::string err;
-    auto cleanup = [&]() {
-      for (auto &s : ckpt) free_state(s);
-      for (auto &s : seg) free_state(s);
-      free_state(work);
-      for (void *q : {(void *)csub, (void *)lam_c, (void *)lam_c2, (void *)lam_phi,
-                      (void *)lam_phi2, (void *)lam_pf, (void *)lam_pf2, (void *)v2_lam,
-                      (void *)accD, (void *)accMS, (void *)part, (void *)target, (void *)v2_id,
-                      (void *)v2_n})
-        if (q) hipFree(q);
-    };
-    hipError_t e = hipSuccess;
-    for (auto &s : ckpt)
-      if (e == hipSuccess) e = alloc_state(s, N, KI, K, P);
-    for (auto &s : seg)
-      if (e == hipSuccess) e = alloc_state(s, N, KI, K, P);
-    if (e == hipSuccess) e = alloc_state(work, N, KI, K, P);
-    if (e == hipSuccess) e = hipMalloc(&csub, size_t(N) * 8 * (nsub + 1));
-    for (double **q : {&lam_c, &lam_c2, &lam_pf, &lam_pf2})
-      if (e == hipSuccess) e = hipMalloc(q, size_t(N) * 8 * (q == &lam_pf || q == &lam_pf2 ? P : 1));
-    if (e == hipSuccess) e = hipMalloc(&lam_phi, size_t(N) * K * 8);
-    if (e == hipSuccess) e = hipMalloc(&lam_phi2, size_t(N) * K * 8);
-    if (e == hipSuccess) e = hipMalloc(&v2_lam, size_t(N) * kStencilMax * 8);
-    if (e == hipSuccess) e = hipMalloc(&v2_id, size_t(N) * kStencilMax * 4);
-    if (e == hipSuccess) e = hipMalloc(&v2_n, size_t(N));
-    if (e == hipSuccess) e = hipMalloc(&accD, size_t(gb) * 3 * 8);
-    if (e == hipSuccess) e = hipMalloc(&accMS, size_t(gs) * 2 * 8);
-    if (e == hipSuccess) e = hipMalloc(&part, size_t(gb) * 8);
-    if (e == hipSuccess && sp->target_conc) e = hipMalloc(&target, size_t(N) * 8);
-    if (e != hipSuccess) {
-      cleanup();
-      return handle_fail(h, PF_ERR_OOM, "adjoint buffers: " + std::string(hipGetErrorString(e)));
+    const size_t n8 = size_t(N) * 8;
+
+    Arena ar;
+    std::vector<DevState> ckpt;
+    for (int q = 0; q < nseg; ++q) ckpt.push_back(ar.state(N, KI, K, P));
+    const DevState fin = ar.state(N, KI, K, P);
+    double *lam_c = ar.get<double>(N), *lam_c2 = ar.get<double>(N);
+    double *lam_pf = ar.get<double>(size_t(N) * P), *lam_pf2 = ar.get<double>(size_t(N) * P);
+    double *lam_phi = ar.get<double>(size_t(N) * K), *lam_phi2 = ar.get<double>(size_t(N) * K);
+    double *v2_lam = ar.get<double>(size_t(N) * kStencilMax);
+    uint32_t *v2_id = ar.get<uint32_t>(size_t(N) * kStencilMax);
+    uint8_t *v2_n = ar.get<uint8_t>(N);
+    double *accD = ar.get<double>(gb), *accMS = ar.get<double>(size_t(gs) * 2);
+    double *part = ar.get<double>(gb);
+    double *target = sp->target_conc ? ar.get<double>(N) : nullptr;
+    std::vector<StepData> seg(CK);
+    for (auto &sd : seg) sd.in = ar.state(N, KI, K, P);
+    // Sub-step concentrations for a whole segment when they fit (every step is then recomputed
+    // once in the reverse sweep), else for one step (steps are recomputed from the segment's
+    // checkpoint to collect their inputs, then once more each).
+    const size_t per_step = n8 * (nsub + 1) + n8 * P + size_t(N);
+    size_t mem_free = 0, mem_total = 0;
+    if (ar.err == hipSuccess) ar.err = hipMemGetInfo(&mem_free, &mem_total);
+    const char *env = std::getenv("PFAMD_ADJ_RECOMPUTE");
+    const bool force_step = env && std::strcmp(env, "step") == 0;
+    const bool store_segment = !force_step && double(per_step) * CK < 0.8 * double(mem_free);
+    const int nstore = store_segment ? CK : 1;
+    for (int q = 0; q < nstore; ++q) {
+      seg[q].pf_new = ar.get<double>(size_t(N) * P);
+      seg[q].flag = ar.get<uint8_t>(N);
+      seg[q].csub = ar.get<double>(size_t(N) * (nsub + 1));
     }
-    auto copy_state = [&](DevState &dst, const DevState &src) {
+    if (ar.err != hipSuccess)
+      return handle_fail(h, PF_ERR_OOM,
+                         std::string("adjoint buffers: ") + hipGetErrorString(ar.err));
+
+    auto copy_state = [&](const DevState &dst, const DevState &src) {
       hipMemcpyAsync(dst.id, src.id, size_t(N) * KI * 2, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(dst.phi, src.phi, size_t(N) * K * 8, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(dst.pf, src.pf, size_t(N) * P * 8, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(dst.c, src.c, size_t(N) * 8, hipMemcpyDeviceToDevice, st);
+      hipMemcpyAsync(dst.phi, src.phi, n8 * K, hipMemcpyDeviceToDevice, st);
+      hipMemcpyAsync(dst.pf, src.pf, n8 * P, hipMemcpyDeviceToDevice, st);
+      hipMemcpyAsync(dst.c, src.c, n8, hipMemcpyDeviceToDevice, st);
     };
-    // the handle's current state <-> DevState
-    auto from_handle = [&](DevState &dst) {
-      HandleView w = handle_view(h);
-      DevState src{w.id, static_cast<double *>(w.phi), static_cast<double *>(w.pf),
-                   static_cast<double *>(w.c0)};
-      copy_state(dst, src);
+    auto on_handle = [&]() {
+      const HandleView w = handle_view(h);
+      DevState d;
+      d.id = w.id;
+      d.phi = static_cast<double *>(w.phi);
+      d.pf = static_cast<double *>(w.pf);
+      d.c = static_cast<double *>(w.c0);
+      return d;
     };
-    auto to_handle = [&](const DevState &src) {
-      HandleView w = handle_view(h);
-      DevState dst{w.id, static_cast<double *>(w.phi), static_cast<double *>(w.pf),
-                   static_cast<double *>(w.c0)};
-      copy_state(dst, src);
+    auto save = [&](const DevState &dst) { copy_state(dst, on_handle()); };
+    auto load = [&](const DevState &src) {
+      copy_state(on_handle(), src);
       handle_state_changed(h);
     };
-    if (target)
-      hipMemcpyAsync(target, sp->target_conc, size_t(N) * 8, hipMemcpyHostToDevice, st);
-    // ---- forward with checkpoints ----
-    auto t0 = std::chrono::steady_clock::now();
-    double loss = 0.0;
-    std::vector<double> frac(std::max(sp->n_samples, 0), 0.0);
     auto sample_index = [&](int step) -> int {
       for (int q = 0; q < sp->n_samples; ++q)
         if (sp->sample_steps[q] == step) return q;
       return -1;
     };
+    auto launch_ok = [&]() -> int {
+      const hipError_t e = hipGetLastError();
+      return e == hipSuccess ? PF_OK
+                             : handle_fail(h, PF_ERR_HIP, std::string("adjoint: ") +
+                                                              hipGetErrorString(e));
+    };
+
+    // ---- forward with checkpoints ----
+    if (target) hipMemcpyAsync(target, sp->target_conc, n8, hipMemcpyHostToDevice, st);
+    const auto t0 = std::chrono::steady_clock::now();
+    double loss = 0.0;
+    std::vector<double> frac(sp->n_samples, 0.0);
     int rc = PF_OK;
     for (int k = 0; k < nsteps && rc == PF_OK; ++k) {
-      if (k % CK == 0) from_handle(ckpt[k / CK]);
+      if (k % CK == 0) save(ckpt[k / CK]);
       rc = pf_step(h, 1);
       const int q = sample_index(k + 1);
       if (rc == PF_OK && q >= 0) {
@@ -550,98 +597,98 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         loss += d * d;
       }
     }
-    DevState final_state;
-    if (rc == PF_OK) {
-      e = alloc_state(final_state, N, KI, K, P);
-      if (e == hipSuccess) from_handle(final_state);
-    }
-    if (rc != PF_OK || e != hipSuccess) {
-      free_state(final_state);
-      cleanup();
-      return rc != PF_OK ? rc : handle_fail(h, PF_ERR_OOM, "adjoint final state");
-    }
-    // loss concentration term and its seed
-    hipMemsetAsync(lam_c, 0, size_t(N) * 8, st);
-    hipMemsetAsync(lam_phi, 0, size_t(N) * K * 8, st);
-    hipMemsetAsync(lam_pf, 0, size_t(N) * P * 8, st);
-    hipMemsetAsync(accD, 0, size_t(gb) * 3 * 8, st);
-    hipMemsetAsync(accMS, 0, size_t(gs) * 2 * 8, st);
+    if (rc != PF_OK) return rc;
+    save(fin);
+    hipMemsetAsync(lam_c, 0, n8, st);
+    hipMemsetAsync(lam_phi, 0, n8 * K, st);
+    hipMemsetAsync(lam_pf, 0, n8 * P, st);
+    hipMemsetAsync(accD, 0, size_t(gb) * 8, st);
+    hipMemsetAsync(accMS, 0, size_t(gs) * 16, st);
     if (target && sp->weight_conc != 0.0) {
-      hipLaunchKernelGGL(adj_seed_conc_kernel, dim3(gb), dim3(256), 0, st, final_state.c, target,
+      hipLaunchKernelGGL(adj_seed_conc_kernel, dim3(gb), dim3(256), 0, st, fin.c, target,
                          sp->weight_conc, lam_c, part, N);
       std::vector<double> hp(gb);
       hipMemcpyAsync(hp.data(), part, size_t(gb) * 8, hipMemcpyDeviceToHost, st);
-      hipStreamSynchronize(st);
+      if (hipStreamSynchronize(st) != hipSuccess) return launch_ok();
       for (double x : hp) loss += x;
     }
-    hipStreamSynchronize(st);
-    auto t1 = std::chrono::steady_clock::now();
+    if (hipStreamSynchronize(st) != hipSuccess) return launch_ok();
+    const auto t1 = std::chrono::steady_clock::now();
+
     // ---- reverse ----
+    // recompute the step whose input the handle holds into slot q
+    auto recompute = [&](int q) -> int {
+      StepData &sd = seg[q];
+      int r = pf_phase_field_sweep(h);
+      if (r != PF_OK) return r;
+      const DevState cur = on_handle();
+      hipMemcpyAsync(sd.pf_new, cur.pf, n8 * P, hipMemcpyDeviceToDevice, st);
+      hipLaunchKernelGGL((classify_kernel<double, P, NF, NOF>), dim3(gb), dim3(256), 0, st,
+                         sd.pf_new, sd.flag, v.nx, v.ny, v.nz, N, v.bulk_mask);
+      hipMemcpyAsync(sd.csub, cur.c, n8, hipMemcpyDeviceToDevice, st);
+      for (int s = 0; s < nsub && r == PF_OK; ++s)
+        r = handle_diffuse_io(h, sd.csub + size_t(N) * s, sd.csub + size_t(N) * (s + 1));
+      if (r != PF_OK) return r;
+      // the step's output carbon, the next step's input
+      hipMemcpyAsync(cur.c, sd.csub + size_t(N) * nsub, n8, hipMemcpyDeviceToDevice, st);
+      return launch_ok();
+    };
+    // reverse step k from slot q (sub-step data) and its input state
+    auto reverse = [&](int q, int k, const DevState &in) -> int {
+      const StepData &sd = seg[q];
+      const int qs = sample_index(k + 1);
+      if (qs >= 0) {
+        const double g = 2.0 * (frac[qs] - sp->target_fraction[qs]) / double(N);
+        hipLaunchKernelGGL(adj_seed_fraction_kernel, dim3(gb), dim3(256), 0, st, lam_pf, P, g, N);
+      }
+      for (int s = nsub - 1; s >= 0; --s) {
+        hipLaunchKernelGGL((adj_diffuse_kernel<P, NF, NOF>), dim3(gb), dim3(256), 0, st,
+                           sd.csub + size_t(N) * s, sd.pf_new, sd.flag, lam_c, lam_c2, lam_pf,
+                           accD, dc, v.nx, v.ny, v.nz, N);
+        std::swap(lam_c, lam_c2);
+      }
+      hipMemsetAsync(lam_phi2, 0, n8 * K, st);
+      hipMemsetAsync(lam_pf2, 0, n8 * P, st);
+      hipLaunchKernelGGL((adj_sweep_kernel<K, P, NF, NOF>), dim3(gs), dim3(kAdjBS), 0, st, in.id,
+                         in.phi, in.pf, in.c, lam_phi, lam_pf, lam_phi2, lam_pf2, lam_c, v2_id,
+                         v2_lam, v2_n, accMS, cs, v.nx, v.ny, v.nz, N);
+      hipLaunchKernelGGL((adj_gather_kernel<K, NF, NOF>), dim3(gb), dim3(256), 0, st, in.id,
+                         v2_id, v2_lam, v2_n, lam_phi2, v.nx, v.ny, v.nz, N);
+      std::swap(lam_phi, lam_phi2);
+      std::swap(lam_pf, lam_pf2);
+      return launch_ok();
+    };
     for (int sg = nseg - 1; sg >= 0 && rc == PF_OK; --sg) {
       const int k0 = sg * CK, k1 = std::min(nsteps, k0 + CK);
-      // recompute the segment, keeping each step's input state
-      to_handle(ckpt[sg]);
-      for (int k = k0; k < k1 && rc == PF_OK; ++k) {
-        from_handle(seg[k - k0]);
-        if (k + 1 < k1) rc = pf_step(h, 1);
-      }
-      for (int k = k1 - 1; k >= k0 && rc == PF_OK; --k) {
-        // loss seed on value_phase after step k
-        const int q = sample_index(k + 1);
-        if (q >= 0) {
-          const double g = 2.0 * (frac[q] - sp->target_fraction[q]) / double(N);
-          hipLaunchKernelGGL(adj_seed_fraction_kernel, dim3(gb), dim3(256), 0, st, lam_pf, P, g,
-                             N);
+      load(ckpt[sg]);
+      if (store_segment) {
+        for (int k = k0; k < k1 && rc == PF_OK; ++k) {
+          save(seg[k - k0].in);
+          rc = recompute(k - k0);
         }
-        // recompute the step: sweep, then the sub-steps storing every concentration
-        to_handle(seg[k - k0]);
-        rc = pf_phase_field_sweep(h);
-        if (rc != PF_OK) break;
-        HandleView w = handle_view(h);
-        const double *pf_new = static_cast<const double *>(w.pf);
-        hipMemcpyAsync(csub, w.c0, size_t(N) * 8, hipMemcpyDeviceToDevice, st);
-        for (int s = 0; s < nsub && rc == PF_OK; ++s) {
-          rc = pf_diffuse_substeps(h, 0, 1, p.dt_d[0]);
-          HandleView w2 = handle_view(h);
-          hipMemcpyAsync(csub + size_t(N) * (s + 1), w2.c0, size_t(N) * 8,
-                         hipMemcpyDeviceToDevice, st);
+        for (int k = k1 - 1; k >= k0 && rc == PF_OK; --k) rc = reverse(k - k0, k, seg[k - k0].in);
+      } else {
+        for (int k = k0; k < k1 && rc == PF_OK; ++k) {
+          save(seg[k - k0].in);
+          if (k + 1 < k1) rc = pf_step(h, 1);
         }
-        if (rc != PF_OK) break;
-        // reverse sub-steps: lam_pf collects the value_phase adjoint of the sweep output
-        for (int s = nsub - 1; s >= 0; --s) {
-          hipLaunchKernelGGL((adj_diffuse_kernel<P, NF, NOF>), dim3(gb), dim3(256), 0, st,
-                             csub + size_t(N) * s, pf_new, lam_c, lam_c2, lam_pf, accD, dc,
-                             v.nx, v.ny, v.nz, N);
-          std::swap(lam_c, lam_c2);
-        }
-        // reverse sweep: sweep input is seg[k - k0]
-        const DevState &in = seg[k - k0];
-        hipMemsetAsync(lam_phi2, 0, size_t(N) * K * 8, st);
-        hipMemsetAsync(lam_pf2, 0, size_t(N) * P * 8, st);
-        hipLaunchKernelGGL((adj_sweep_kernel<K, P, NF, NOF>), dim3(gs), dim3(kAdjBS), 0, st,
-                           in.id, in.phi, in.pf, in.c, lam_phi, lam_pf, lam_phi2, lam_pf2, lam_c,
-                           v2_id, v2_lam, v2_n, accMS, cs, v.nx, v.ny, v.nz, N);
-        hipLaunchKernelGGL((adj_gather_kernel<K, NF, NOF>), dim3(gb), dim3(256), 0, st, in.id,
-                           v2_id, v2_lam, v2_n, lam_phi2, v.nx, v.ny, v.nz, N);
-        std::swap(lam_phi, lam_phi2);
-        std::swap(lam_pf, lam_pf2);
-        e = hipGetLastError();
-        if (e != hipSuccess) {
-          rc = handle_fail(h, PF_ERR_HIP, std::string("adjoint: ") + hipGetErrorString(e));
-          break;
+        for (int k = k1 - 1; k >= k0 && rc == PF_OK; --k) {
+          load(seg[k - k0].in);
+          rc = recompute(0);
+          if (rc == PF_OK) rc = reverse(0, k, seg[k - k0].in);
         }
       }
     }
     if (rc == PF_OK) {
-      std::vector<double> hD(size_t(gb) * 3), hMS(size_t(gs) * 2);
+      std::vector<double> hD(gb), hMS(size_t(gs) * 2);
       hipMemcpyAsync(hD.data(), accD, hD.size() * 8, hipMemcpyDeviceToHost, st);
       hipMemcpyAsync(hMS.data(), accMS, hMS.size() * 8, hipMemcpyDeviceToHost, st);
-      if (out->d_conc0)
-        hipMemcpyAsync(out->d_conc0, lam_c, size_t(N) * 8, hipMemcpyDeviceToHost, st);
-      e = hipStreamSynchronize(st);
+      if (out->d_conc0) hipMemcpyAsync(out->d_conc0, lam_c, n8, hipMemcpyDeviceToHost, st);
+      const hipError_t e = hipStreamSynchronize(st);
       if (e != hipSuccess) rc = handle_fail(h, PF_ERR_HIP, hipGetErrorString(e));
+      const auto t2 = std::chrono::steady_clock::now();
       double dD = 0.0, dM = 0.0, dS = 0.0;
-      for (unsigned b = 0; b < gb; ++b) dD += hD[size_t(b) * 3];
+      for (double x : hD) dD += x;
       for (unsigned b = 0; b < gs; ++b) {
         dM += hMS[size_t(b) * 2];
         dS += hMS[size_t(b) * 2 + 1];
@@ -650,16 +697,14 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       out->d_mobility = dM;
       out->d_sigma = dS;
       out->d_diff_gamma = dD;
-      for (int q = 0; q < sp->n_samples && q < 64; ++q) out->fraction[q] = frac[q];
-      auto t2 = std::chrono::steady_clock::now();
+      for (int q = 0; q < sp->n_samples; ++q) out->fraction[q] = frac[q];
       out->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
       out->reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     }
     // leave the final forward state on the handle
-    to_handle(final_state);
-    hipStreamSynchronize(st);
-    free_state(final_state);
-    cleanup();
+    load(fin);
+    const hipError_t e = hipStreamSynchronize(st);
+    if (rc == PF_OK && e != hipSuccess) rc = handle_fail(h, PF_ERR_HIP, hipGetErrorString(e));
     return rc;
   }
 };

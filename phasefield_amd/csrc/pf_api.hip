@@ -62,6 +62,7 @@ struct pf_handle {
   bool flag_valid = false;   // flags describe the current value_phase and stencil
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
+  unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
 };
 
 namespace {
@@ -123,6 +124,26 @@ template <class A> DiffConsts<A> make_diff_consts(const pf_params &p, int s, dou
   }
   d.r = A(factor);
   return d;
+}
+
+// Bit p (1-based phase) when every solute's partition table has k(p,p) == 1 and c(p,p) == 0 in
+// the form's arithmetic type: the condition under which the bulk-cell reduction of the diffusion
+// sub-step is exact.
+void update_bulk_mask(pf_handle *h) {
+  unsigned m = 0;
+  for (int p = 0; p < h->d.nphase; ++p) {
+    bool ok = true;
+    for (int s = 0; s < h->d.nsolute; ++s) {
+      const double k = h->p.part_k[s][p][p], c = h->p.part_c[s][p][p];
+      if (is_mixed(h))
+        ok = ok && float(k) == 1.0f && float(c) == 0.0f;
+      else
+        ok = ok && k == 1.0 && c == 0.0;
+    }
+    if (ok) m |= 1u << (p + 1);
+  }
+  h->bulk_mask = m;
+  h->flag_valid = false;
 }
 
 // ---- launch helpers -----------------------------------------------------------------------------
@@ -207,13 +228,11 @@ DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk) {
   return g;
 }
 
+// one sub-step from `cin` into `cout` with the handle's kernel choice and bulk flags
 template <class F, int P, int NF, bool NOF>
-hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &dc) {
+hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
+                             const typename F::C *cin, typename F::C *cout) {
   using T = typename F::T;
-  using C = typename F::C;
-  const int a = h->ccur[s], b = 1 - a;
-  const C *cin = static_cast<const C *>(h->conc[s][a]);
-  C *cout = static_cast<C *>(h->conc[s][b]);
   const T *pf = static_cast<const T *>(h->pfrac);
   if (h->diff_variant == 1) {
     const DiffGeom g = diff_geom(h, kDiffTY, 2048, 16);
@@ -223,7 +242,7 @@ hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &
   } else {
     if (!h->flag_valid || h->flag_nf != NF) {
       hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(blocks_for(h->N)), dim3(256), 0,
-                         h->stream, pf, h->flag, h->d.nx, h->d.ny, h->d.nz, h->N);
+                         h->stream, pf, h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask);
       h->flag_valid = true;
       h->flag_nf = NF;
     }
@@ -247,8 +266,17 @@ hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &
     else
       go(std::integral_constant<int, 4>());
   }
-  h->ccur[s] = b;
   return hipGetLastError();
+}
+
+template <class F, int P, int NF, bool NOF>
+hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &dc) {
+  using C = typename F::C;
+  const int a = h->ccur[s], b = 1 - a;
+  const hipError_t e = launch_diffuse_io<F, P, NF, NOF>(
+      h, dc, static_cast<const C *>(h->conc[s][a]), static_cast<C *>(h->conc[s][b]));
+  h->ccur[s] = b;
+  return e;
 }
 
 template <class F, int P>
@@ -410,6 +438,7 @@ int pf_set_params(pf_handle *h, const pf_params *p) {
     if (p->n_sub[s] < 0) return fail(h, PF_ERR_INVALID, "negative sub-step count");
   h->p = *p;
   h->have_params = true;
+  update_bulk_mask(h);
   return PF_OK;
 }
 
@@ -1036,6 +1065,7 @@ void GPU_Diffusion_solver(double *hdiffusivity, double *hpartition_k, double *hp
   p.dt_d[0] = dt / num_loop;
   h->p = p;
   h->have_params = true;
+  update_bulk_mask(h);
   const long N = h->N;
   std::vector<int32_t> idx(size_t(N) * 4, 0);
   std::vector<double> val(size_t(N) * 4, 0.0);
@@ -1090,9 +1120,30 @@ HandleView handle_view(pf_handle *h) {
   v.phi = h->phi[h->cur];
   v.pf = h->pfrac;
   v.c0 = h->conc[0][h->ccur[0]];
+  v.bulk_mask = h->bulk_mask;
   return v;
 }
 int handle_fail(pf_handle *h, int code, const std::string &msg) { return fail(h, code, msg); }
+
+namespace {
+template <int P>
+hipError_t diffuse_io_p(pf_handle *h, const DiffConsts<double> &dc, const double *cin,
+                        double *cout) {
+  if (h->NF == 4) return launch_diffuse_io<FormF64, P, 4, false>(h, dc, cin, cout);
+  if (h->d.bc_z == PF_BC_NOFLUX) return launch_diffuse_io<FormF64, P, 6, true>(h, dc, cin, cout);
+  return launch_diffuse_io<FormF64, P, 6, false>(h, dc, cin, cout);
+}
+}  // namespace
+
+int handle_diffuse_io(pf_handle *h, const double *cin, double *cout) {
+  // the expression pf_step uses for the carbon sub-step factor (bit-identical results)
+  const double factor = h->p.dt_d[0] / (h->p.dx * h->p.dx);
+  const DiffConsts<double> dc = make_diff_consts<double>(h->p, 0, factor);
+  const hipError_t e = h->d.nphase == 2 ? diffuse_io_p<2>(h, dc, cin, cout)
+                                        : diffuse_io_p<3>(h, dc, cin, cout);
+  if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("diffuse: ") + hipGetErrorString(e));
+  return PF_OK;
+}
 void handle_state_changed(pf_handle *h) { h->flag_valid = false; }
 PFConsts<double> handle_pf_consts(const pf_handle *h) { return make_pf_consts<double>(h); }
 DiffConsts<double> handle_diff_consts(const pf_handle *h, int solute, double factor) {

@@ -228,17 +228,18 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
 // ------------------------------------------------------------------------------------------------
 // Bulk classification, run once per phase-field sweep (value_phase is constant over the diffusion
 // sub-cycle): flag = p (1-based) when the cell and all 2*dims stencil neighbours hold exactly the
-// unit vector e_p in value_phase, else 0.  For such a cell the reference's sub-step reduces
-// exactly to del = sum_faces (c_nb - c) * D_p:
+// unit vector e_p in value_phase and bit p of `mask` is set, else 0.  The host sets bit p when
+// every solute's partition table has k(p,p) == 1 and c(p,p) == 0.  For such a cell the
+// reference's sub-step reduces exactly to del = sum_faces (c_nb - c) * D_p:
 //   weights are sqrt(1*1) = 1 for phase p and sqrt(0*0) = 0 for the others, whose terms are
 //   0 * finite = +-0 and leave the running sum unchanged (it starts at +0 and can never become
-//   -0 by round-to-nearest addition); the partition of phase p is (c - t2)/t1 with t1 = 1 and
-//   t2 = +-0, i.e. exactly c.
+//   -0 by round-to-nearest addition); the partition of phase p is (c - t2)/t1 with
+//   t1 = 1*k(p,p) + sum 0*k(p,j) = 1 and t2 = 1*c(p,p) + sum 0*c(p,j) = +-0, i.e. exactly c.
 template <class T, int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfrac,
                                                        uint8_t *__restrict__ flag, const int nx,
                                                        const int ny, const int nz,
-                                                       const long ncell) {
+                                                       const long ncell, const unsigned mask) {
   const long cell = long(blockIdx.x) * 256 + threadIdx.x;
   if (cell >= ncell) return;
   const int x = int(cell % nx);
@@ -260,6 +261,7 @@ __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfr
     return (ones == 1 && zeros == P - 1) ? p : 0;
   };
   int p = unit(cell);
+  if (!((mask >> p) & 1u)) p = 0;
   if (p != 0) {
     const long plane = long(nx) * ny;
     long nb[6];
@@ -471,13 +473,13 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
 }
 
 // explicit instantiations used by the host dispatch (pf_api.hip)
-#define PF_INST_CLASSIFY(T, P)                                                               \
-  template __global__ void classify_kernel<T, P, 4, false>(const T *, uint8_t *, int, int, int, \
-                                                           long);                              \
-  template __global__ void classify_kernel<T, P, 6, false>(const T *, uint8_t *, int, int, int, \
-                                                           long);                              \
-  template __global__ void classify_kernel<T, P, 6, true>(const T *, uint8_t *, int, int, int,  \
-                                                          long);
+#define PF_INST_CLASSIFY(T, P)                                                                \
+  template __global__ void classify_kernel<T, P, 4, false>(const T *, uint8_t *, int, int, int,  \
+                                                           long, unsigned);                     \
+  template __global__ void classify_kernel<T, P, 6, false>(const T *, uint8_t *, int, int, int,  \
+                                                           long, unsigned);                     \
+  template __global__ void classify_kernel<T, P, 6, true>(const T *, uint8_t *, int, int, int,   \
+                                                          long, unsigned);
 PF_INST_CLASSIFY(double, 2)
 PF_INST_CLASSIFY(double, 3)
 PF_INST_CLASSIFY(float, 2)

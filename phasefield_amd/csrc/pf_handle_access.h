@@ -18,6 +18,7 @@ struct HandleView {
   void *phi;     // current slot values
   void *pf;      // value_phase
   void *c0;      // current carbon concentration
+  unsigned bulk_mask;  // phases (bit p) whose bulk cells take the exact reduced sub-step
 };
 
 HandleView handle_view(pf_handle *h);
@@ -27,3 +28,6 @@ int handle_fail(pf_handle *h, int code, const std::string &msg);
 void handle_state_changed(pf_handle *h);
 pfamd::PFConsts<double> handle_pf_consts(const pf_handle *h);
 pfamd::DiffConsts<double> handle_diff_consts(const pf_handle *h, int solute, double factor);
+// one FP64 carbon sub-step from cin into cout with the step's own kernel, flags and factor
+// (value_phase and bulk flags of the handle's current state)
+int handle_diffuse_io(pf_handle *h, const double *cin, double *cout);

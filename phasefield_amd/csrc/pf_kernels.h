@@ -38,7 +38,8 @@ __global__ void diffuse_linear_kernel(const typename F::C *__restrict__ cin,
 
 template <class T, int P, int NF, bool NOFLUX>
 __global__ void classify_kernel(const T *__restrict__ pfrac, uint8_t *__restrict__ flag,
-                                const int nx, const int ny, const int nz, const long ncell);
+                                const int nx, const int ny, const int nz, const long ncell,
+                                const unsigned mask);
 
 template <class F, int P, int NF, bool NOFLUX, int BY>
 __global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,

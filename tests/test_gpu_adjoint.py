@@ -132,9 +132,10 @@ def test_adjoint_matches_central_differences(tmp_path):
     assert abs(fd - gc @ d) <= 1e-4 * abs(fd) + 1e-10
 
 
-def test_checkpoint_spacing_and_final_state(tmp_path):
-    """Gradients do not depend on the checkpoint spacing (the recomputation is deterministic) and
-    the domain is left in the plain forward run's final state."""
+def test_checkpoint_spacing_and_final_state(tmp_path, monkeypatch):
+    """Gradients do not depend on the checkpoint spacing or on whether a segment's sub-step
+    concentrations are stored or recomputed per step (the recomputation is deterministic), and the
+    domain is left in the plain forward run's final state."""
     shape, bc, K, P = (20, 18, 10), "periodic", 4, 2
     banner("adjoint checkpoint spacing")
     rng = np.random.default_rng(5)
@@ -143,7 +144,8 @@ def test_checkpoint_spacing_and_final_state(tmp_path):
     samples, targets, wc, target_c = _loss_spec(st, rng, f0)
     g = _gpu_domain(shape, 0, K, P, pp)
     res = []
-    for ck in (1, 2, STEPS):
+    for ck, mode in ((1, "segment"), (2, "segment"), (STEPS, "segment"), (2, "step")):
+        monkeypatch.setenv("PFAMD_ADJ_RECOMPUTE", mode)
         g.upload(*st)
         r = g.adjoint(STEPS, samples, targets, wc, target_c, checkpoint_every=ck)
         res.append(r)

@@ -131,6 +131,18 @@ def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows):
     assert_same(gs, os_, 4, f"{kernel}/{rows}")
 
 
+def test_bulk_reduction_needs_identity_partition(tmp_path):
+    """With k(2,2) != 1 the partition of a pure ferrite cell is not its total concentration, so the
+    bulk-cell reduction must not be taken for ferrite: results stay bitwise equal to the oracle."""
+    banner("bulk reduction guard")
+    m, p, st = _setup_state(tmp_path, nx=40, ny=36, nz=20, grains=5, nuclei=4)
+    pp = oracle.as_orc_params(p)
+    pp.part_k[0][1][1] = 1.1
+    gs, os_ = run_pair((40, 36, 20), 4, 2, 1, 0, oracle.ORC_FP64, st, pp,
+                       lambda g, q: g.step(2), lambda o, q: o.step(2))
+    assert_same(gs, os_, 4, "k(2,2)=1.1")
+
+
 def test_two_solutes_bitwise(tmp_path):
     """Fe-C-Mn: carbon sub-cycled, manganese one explicit update per step."""
     banner("two solutes")
