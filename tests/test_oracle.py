@@ -407,3 +407,97 @@ def test_dual_derivatives_match_central_differences(tmp_path):
         scale_f = max(abs(fd_f), 1e-12)
         assert abs(df[k] - fd_f) <= 1e-5 * scale_f + 1e-12, (k, df[k], fd_f)
         assert abs(ad_c - fd_c) <= 1e-5 * max(abs(fd_c), 1e-12) + 1e-12, (k, ad_c, fd_c)
+
+
+# ---- flat interfaces: closed forms of the continuum model pin the pair term ----------------------
+# For two grains with phi_a + phi_b = 1 the sweep is explicit Euler for
+#   dphi/dt = M [sigma (lap(phi) + pi^2/eta^2 (phi - 1/2)) + pi/eta sqrt(phi (1 - phi)) dG]
+# ((S_a phi_b - S_b phi_a)/dx^2 is the discrete Laplacian when phi_a + phi_b = 1), whose
+# travelling-wave solution is the sine profile phi = 1/2 + 1/2 sin(pi (x - v t)/eta) of width eta
+# moving at v = M dG, for any dG.
+
+def _slab(nx, ny, K, lo, hi, width, id_in, id_out):
+    """Grain id_in on x in [lo, hi] with sine-profile interfaces of the given width, id_out
+    elsewhere; uniform along y."""
+    x = np.arange(nx, dtype=float)
+
+    def prof(d):
+        return np.where(d >= width / 2, 1.0,
+                        np.where(d <= -width / 2, 0.0, 0.5 + 0.5 * np.sin(np.pi * d / width)))
+
+    a = np.minimum(prof(x - lo), prof(hi - x))
+    ids = np.zeros((ny, nx, K), np.int32)
+    val = np.zeros((ny, nx, K))
+    for i in range(nx):
+        ent = sorted([(v, g) for v, g in ((a[i], id_in), (1 - a[i], id_out)) if v > 0],
+                     key=lambda t: -t[0])
+        for s, (v, g) in enumerate(ent):
+            ids[:, i, s] = g
+            val[:, i, s] = v
+    return a, ids.reshape(-1), val.reshape(-1)
+
+
+def _grain_profile(st, gid, nx, ny, K):
+    ii = st["index"].reshape(ny, nx, K)
+    vv = st["value"].reshape(ny, nx, K)
+    return np.where(ii == gid, vv, 0.0).sum(-1)[0]
+
+
+@pytest.mark.parametrize("start_width", [1.5, 0.6])
+def test_flat_interface_relaxes_to_sine_profile(params3d, start_width):
+    """Two ferrite grains (dG = 0): a profile of the wrong width relaxes onto the width-eta sine
+    profile and the grain keeps its volume (the two interfaces are mirror images)."""
+    nx, ny, K = 64, 2, 4
+    q = oracle.as_orc_params(params3d)
+    q.dims = 2
+    w = q.eta / q.dx
+    a0, ids, val = _slab(nx, ny, K, 16, 48, w * start_width, 200001, 200002)
+    ref, _, _ = _slab(nx, ny, K, 16, 48, w, 200001, 200002)
+    phase = np.zeros((nx * ny, 3))
+    phase[:, 1] = 1.0
+    o = oracle.OracleDomain(nx, ny, 1, K, 1)
+    o.set_params(q)
+    o.set_state(ids, val, phase.reshape(-1), np.full(nx * ny, 0.02))
+    for _ in range(600):
+        o.sweep()
+    a = _grain_profile(o.get_state(), 200001, nx, ny, K)
+    assert np.max(np.abs(a0 - ref)) > 0.1  # the start is far from the equilibrium shape
+    assert np.max(np.abs(a - ref)) < 0.01
+    assert abs(a.sum() - ref.sum()) < 1e-9
+
+
+@pytest.mark.parametrize("dG", [0.25e-12, 0.5e-12])
+def test_flat_interface_velocity(params3d, dG):
+    """Austenite/ferrite slab with a constant driving force (partition k = 1, c = 0, so the local
+    concentration is the total one; dG(1,2) = +dG, dG(2,1) = -dG): the austenite grows, each front
+    moving at v = M dG.  At eta = 6 dx the discrete front runs 5-6% slow."""
+    nx, ny, K = 64, 2, 4
+    r = oracle.as_orc_params(params3d)
+    r.dims = 2
+    for i in range(3):
+        for j in range(3):
+            r.mobility[i][j] = r.sigma[i][j] = r.entropy[i][j] = r.eq_conc[i][j] = 0.0
+            r.eq_slope[i][j] = 1.0
+            for s in range(2):
+                r.part_k[s][i][j], r.part_c[s][i][j] = 1.0, 0.0
+    M = 1e12
+    for (i, j) in ((0, 1), (1, 0)):
+        r.mobility[i][j] = M
+        r.sigma[i][j] = 0.5e-12
+    r.entropy[0][1], r.eq_slope[0][1] = dG, 1.0
+    r.entropy[1][0], r.eq_slope[1][0] = dG, -1.0
+    w = r.eta / r.dx
+    a0, ids, val = _slab(nx, ny, K, 16, 48, w, 100001, 200001)
+    phase = np.zeros((nx * ny, 3))
+    phase[:, 0] = np.tile(a0, ny)
+    phase[:, 1] = 1.0 - phase[:, 0]
+    o = oracle.OracleDomain(nx, ny, 1, K, 1)
+    o.set_params(r)
+    o.set_state(ids, val, phase.reshape(-1), np.full(nx * ny, 1.0))
+    vols = []
+    for it in range(600):
+        o.sweep()
+        if it in (199, 599):
+            vols.append(_grain_profile(o.get_state(), 100001, nx, ny, K).sum())
+    v = (vols[1] - vols[0]) * r.dx / 2.0 / (400 * r.pfm_dt)
+    assert 0.9 < v / (M * dG) < 1.0, v / (M * dG)
