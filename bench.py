@@ -62,6 +62,14 @@ def alg_bytes_per_substep(mixed: bool, nphase: int) -> int:
     return nphase * b_phi + 8 + 8
 
 
+def diffusion_kernel_name(mixed: bool, nphase: int, ncell: int) -> str:
+    """The sub-step kernel the library launches for this domain (periodic 3D, default choice)."""
+    form = "FormMixed" if mixed else "FormF64"
+    if ncell >= 1 << 26:
+        return f"diffuse_bulk_kernel<{form},{nphase},6,false,4>"
+    return f"diffuse_linear_kernel<{form},{nphase},6,false>"
+
+
 def timed_run(dom, model, steps: int, warmup: int):
     """warmup untimed steps, then exactly `steps` timed steps with HIP-event kernel timing."""
     for _ in range(warmup):
@@ -82,9 +90,9 @@ def timed_run(dom, model, steps: int, warmup: int):
     return wall, diff_ms, diff_n, pf_ms, pf_n
 
 
-def cpu_baseline(workload: str, tmp: str):
+def cpu_baseline(workload: str, tmp: str, steps: int = 12):
     """Oracle (OpenMP, reference loop schedules) on a 128^3 sample of the workload with the grain
-    and nucleus densities kept: setup, then one full step timed (sweep + all sub-steps)."""
+    and nucleus densities kept: setup, then `steps` full steps timed (sweep + all sub-steps)."""
     import oracle
     from phasefield_amd import Model, configs
 
@@ -103,14 +111,14 @@ def cpu_baseline(workload: str, tmp: str):
     p2 = m.update_parameter()
     od.set_params(p2)
     t0 = time.perf_counter()
-    od.step(1)
+    od.step(steps)
     dt = time.perf_counter() - t0
     n = d.nx * d.ny * d.nz
-    return {"value": n / dt / 1e9, "unit": "Gcell-updates/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": f"1 full step (1 sweep + {p2.n_sub[0]} carbon sub-steps"
-                      f"{' + 1 Mn update' if d.nsolute > 1 else ''}) of a 128^3 cut of the workload "
-                      f"(8 grains, 16 nuclei), FP64, {dt:.1f} s"}
+    return {"value": n * steps / dt / 1e9, "unit": "Gcell-updates/s",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{steps} full steps (1 sweep + {p2.n_sub[0]} carbon sub-steps"
+                      f"{' + 1 Mn update' if d.nsolute > 1 else ''} each) of a 128^3 cut of the "
+                      f"workload (8 grains, 16 nuclei), FP64, {dt:.1f} s"}
 
 
 def adjoint_section(tmp: str, steps: int | None, fd: list):
@@ -206,6 +214,36 @@ def adjoint_section(tmp: str, steps: int | None, fd: list):
     return out
 
 
+def accuracy_run(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, nsolute, steps: int, every: int):
+    """Step both forms `steps` times from the same seeded state (each with its own per-step
+    parameter update) and report the mixed form's error against the FP64 form every `every`
+    steps: austenite fraction, max c per solute, solute-mass drift of each form."""
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(1, steps + 1):
+        for dom, model in ((mixed, m_mixed), (g64, m_fp64)):
+            dom.set_params(model.update_parameter())
+            dom.step(1)
+        if k % every == 0 or k == steps:
+            smx, s64 = mixed.reduce(), g64.reduce()
+            fg64, fgmx = s64["phase_frac"][0], smx["phase_frac"][0]
+            row = {"step": k,
+                   "austenite_fraction": fgmx, "austenite_fraction_fp64": fg64,
+                   "austenite_fraction_rel": abs(fgmx - fg64) / max(abs(fg64), 1e-300),
+                   "max_c_rel": [abs(a - b) / abs(b) for a, b in
+                                 zip(smx["max_conc"][:nsolute], s64["max_conc"][:nsolute])],
+                   "mass_drift_rel_mixed": [abs(a - b) / b for a, b in
+                                            zip(smx["mass"][:nsolute], st0_mixed["mass"][:nsolute])],
+                   "mass_drift_rel_fp64": [abs(a - b) / b for a, b in
+                                           zip(s64["mass"][:nsolute], st0["mass"][:nsolute])]}
+            rows.append(row)
+            log(f"accuracy step {k}: f_gamma rel {row['austenite_fraction_rel']:.3e} max c rel "
+                f"{[f'{x:.2e}' for x in row['max_c_rel']]} mass drift mixed "
+                f"{[f'{x:.2e}' for x in row['mass_drift_rel_mixed']]} "
+                f"({time.perf_counter() - t0:.0f} s)")
+    return rows
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +253,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=0,
+                    help="instead of timing: step mixed and FP64 this many steps and report the "
+                         "mixed form's error against FP64 along the way")
+    ap.add_argument("--accuracy-every", type=int, default=100)
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=0)
     ap.add_argument("--adjoint-fd", default="", metavar="H1,H2,..",
@@ -265,6 +307,15 @@ def main() -> int:
     mixed.copy_from(g64)
     st0 = g64.reduce()
     st0_mixed = mixed.reduce()  # after rounding the FP64 state to FP32
+    if args.accuracy_steps:
+        rows = accuracy_run(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, d.nsolute,
+                            args.accuracy_steps, args.accuracy_every)
+        print(json.dumps({"workload": args.workload, "accuracy": rows,
+                          "mass_fp32_rounding_of_initial_state_rel": [
+                              abs(a - b) / b for a, b in zip(st0_mixed["mass"][: d.nsolute],
+                                                             st0["mass"][: d.nsolute])]}),
+              flush=True)
+        return 0
 
     res = {}
     for name, dom, model in (("mixed", mixed, m_mixed), ("fp64", g64, m_fp64)):
@@ -282,8 +333,7 @@ def main() -> int:
                      "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                                  "kernel": f"diffuse_kernel<{'FormMixed' if name == 'mixed' else 'FormF64'},"
-                                            f"{d.nphase},6>",
+                                  "kernel": diffusion_kernel_name(name == "mixed", d.nphase, N),
                                   "bytes_per_cell": bpc, "cells_per_launch": N,
                                   "launch_us": round(launch_us, 2)}}
         log(f"{name}: {res[name]['ms_per_step']:.2f} ms/step, diffusion {launch_us:.1f} us "

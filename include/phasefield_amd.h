@@ -132,8 +132,10 @@ int pf_diffuse_substeps(pf_handle *h, int solute, int n, double dt_d);
 int pf_reduce(pf_handle *h, pf_stats *out);
 /* Wait for all queued work on the handle's stream. */
 int pf_synchronize(pf_handle *h);
-/* Device time in ms of the dominant kernel launches since the last reset (HIP events on the
- * handle's stream); kind 0 = diffusion sub-step, 1 = phase-field sweep. */
+/* Device time in ms of the dominant kernel launches since the last reset, and how many launches
+ * it covers (one HIP event pair on the handle's stream around each batch of back-to-back
+ * launches); kind 0 = diffusion sub-step (every sub-step of a batch but its first, which also
+ * classifies bulk cells), 1 = phase-field sweep. */
 int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches);
 int pf_kernel_timing_reset(pf_handle *h, int enable);
 /* Size in bytes of one device-side checkpoint; save/restore copy the whole state (device to
