@@ -63,6 +63,9 @@ struct pf_handle {
   bool flag_valid = false;   // flags describe the current value_phase and stencil
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
+  bool diff_split = true;    // full-formula cells from a per-sweep list in their own kernel
+  int *glist = nullptr;      // indices of the cells with flag 0, storage order
+  int *gblock = nullptr;     // per-256-cell-block offsets into glist; [nblocks] = list length
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
 };
 
@@ -245,31 +248,54 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
                        dim3(64, kDiffTY + 3), 0, h->stream, cin, cout, pf, dc, h->d.nx, h->d.ny,
                        h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
   } else {
+    const unsigned nb = blocks_for(h->N);
     if (!h->flag_valid || h->flag_nf != NF) {
-      hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(blocks_for(h->N)), dim3(256), 0,
-                         h->stream, pf, h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask);
+      hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(nb), dim3(256), 0, h->stream, pf,
+                         h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask);
+      if (h->diff_split) {
+        hipLaunchKernelGGL(general_count_kernel, dim3(nb), dim3(256), 0, h->stream, h->flag, h->N,
+                           h->gblock);
+        hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock,
+                           int(nb));
+        hipLaunchKernelGGL(general_scatter_kernel, dim3(nb), dim3(256), 0, h->stream, h->flag,
+                           h->N, h->gblock, h->glist);
+      }
       h->flag_valid = true;
       h->flag_nf = NF;
     }
-    auto go = [&](auto by) {
+    auto go = [&](auto by, auto split) {
       constexpr int BY = decltype(by)::value;
+      constexpr bool SPLIT = decltype(split)::value;
       // z-chunks of ~16 planes: the measured optimum at 512^3 (4 tile rows, 8192+ workgroups)
       const DiffGeom g = diff_geom(h, BY, 131072 / BY, 8);
-      hipLaunchKernelGGL((diffuse_bulk_kernel<F, P, NF, NOF, BY>), dim3(g.nblocks), dim3(64, BY),
-                         0, h->stream, cin, cout, pf, h->flag, dc, h->d.nx, h->d.ny, h->d.nz,
-                         g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+      hipLaunchKernelGGL((diffuse_bulk_kernel<F, P, NF, NOF, BY, SPLIT>), dim3(g.nblocks),
+                         dim3(64, BY), 0, h->stream, cin, cout, pf, h->flag, dc, h->d.nx, h->d.ny,
+                         h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
     };
-    // below ~64M cells the storage-order kernel measured faster (caches hold more of the domain)
-    const int by = h->diff_by >= 0 ? h->diff_by : (h->N < (1L << 26) ? 0 : 4);
-    if (by == 0)
-      hipLaunchKernelGGL((diffuse_linear_kernel<F, P, NF, NOF>), dim3(blocks_for(h->N)), dim3(256),
-                         0, h->stream, cin, cout, pf, h->flag, dc, h->d.nx, h->d.ny, h->d.nz);
-    else if (by == 16)
-      go(std::integral_constant<int, 16>());
-    else if (by == 8)
-      go(std::integral_constant<int, 8>());
-    else
-      go(std::integral_constant<int, 4>());
+    auto go_split = [&](auto split) {
+      constexpr bool SPLIT = decltype(split)::value;
+      // below ~64M cells the storage-order kernel measured faster (caches hold more of the domain)
+      const int by = h->diff_by >= 0 ? h->diff_by : (h->N < (1L << 26) ? 0 : 4);
+      if (by == 0)
+        hipLaunchKernelGGL((diffuse_linear_kernel<F, P, NF, NOF, SPLIT>), dim3(nb), dim3(256), 0,
+                           h->stream, cin, cout, pf, h->flag, dc, h->d.nx, h->d.ny, h->d.nz);
+      else if (by == 16)
+        go(std::integral_constant<int, 16>(), split);
+      else if (by == 8)
+        go(std::integral_constant<int, 8>(), split);
+      else
+        go(std::integral_constant<int, 4>(), split);
+    };
+    if (h->diff_split) {
+      go_split(std::true_type());
+      // the listed cells: a grid-stride pass sized for the whole chip, bounded by the list length
+      const unsigned ng = std::min<unsigned>(nb, 2048);
+      hipLaunchKernelGGL((diffuse_general_kernel<F, P, NF, NOF>), dim3(ng), dim3(256), 0,
+                         h->stream, cin, cout, pf, h->glist, h->gblock + nb, dc, h->d.nx, h->d.ny,
+                         h->d.nz);
+    } else {
+      go_split(std::false_type());
+    }
   }
   return hipGetLastError();
 }
@@ -336,6 +362,8 @@ void free_all(pf_handle *h) {
   }
   if (h->pfrac) hipFree(h->pfrac);
   if (h->flag) hipFree(h->flag);
+  if (h->glist) hipFree(h->glist);
+  if (h->gblock) hipFree(h->gblock);
   if (h->red) hipFree(h->red);
   for (int k = 0; k < 2; ++k)
     for (auto &pr : h->ev[k]) {
@@ -374,6 +402,8 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     return fail(nullptr, PF_ERR_INVALID, "bc_z must be periodic or no-flux");
   const int dims = 3 - (d.nz == 1 ? 1 : 0) - (d.ny == 1 ? 1 : 0);
   if (dims < 2) return fail(nullptr, PF_ERR_INVALID, "one-dimensional domains are not supported");
+  if (long(d.nx) * d.ny * d.nz > 2147483647L)
+    return fail(nullptr, PF_ERR_INVALID, "more than 2^31 - 1 cells in one domain");
   if (d.bc_z == PF_BC_NOFLUX && dims != 3)
     return fail(nullptr, PF_ERR_INVALID, "no-flux z needs a 3D domain");
   int ndev = 0;
@@ -394,6 +424,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if (const char *e = std::getenv("PFAMD_DIFF_KERNEL"))
     h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
   if (const char *e = std::getenv("PFAMD_DIFF_BY")) h->diff_by = std::atoi(e);
+  if (const char *e = std::getenv("PFAMD_DIFF_SPLIT")) h->diff_split = std::atoi(e) != 0;
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {
     if (e == hipSuccess) return PF_OK;
@@ -414,6 +445,10 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if ((rc = chk(hipMalloc(&h->pfrac, N * d.nphase * h->tsz), "alloc value_phase")) != PF_OK)
     return rc;
   if ((rc = chk(hipMalloc(&h->flag, N), "alloc bulk flags")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->gblock, (blocks_for(h->N) + 1) * sizeof(int)), "alloc list offsets")) !=
+      PF_OK)
+    return rc;
   h->red_blocks = int(std::min<long>(1024, std::max<long>(1, (h->N + 4095) / 4096)));
   if ((rc = chk(hipMalloc(&h->red, sizeof(RedPartial) * h->red_blocks), "alloc partials")) != PF_OK)
     return rc;

@@ -79,6 +79,37 @@ template <> __device__ inline float2 conc_update<FormMixed>(float2 c, float del)
   return make_float2(t, (t - c.x) - y);
 }
 
+// The reference's sub-step for one cell from its own and its stencil's stored concentration and
+// value_phase: partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365), update.
+template <class F, int P, int NF>
+__device__ inline typename F::C full_update(const typename F::C *__restrict__ cin,
+                                            const typename F::T *__restrict__ pfrac, long i,
+                                            const long (&nbi)[6],
+                                            const DiffConsts<typename F::A> &dc) {
+  using A = typename F::A;
+  RawCell<F, P> r;
+  r.c = cin[i];
+#pragma unroll
+  for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
+  CellPF<A> me;
+  partition_cell<F, P>(r, dc, me);
+  A del = A(0);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    RawCell<F, P> rn;
+    rn.c = cin[nbi[f]];
+#pragma unroll
+    for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
+    CellPF<A> nb;
+    partition_cell<F, P>(rn, dc, nb);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
+  }
+  del *= dc.r;
+  return conc_update<F>(r.c, del);
+}
+
 template <class F, int P, int NF, bool NOFLUX, int TY>
 __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
@@ -295,7 +326,7 @@ template <> __device__ inline float load_hi<FormMixed>(const float2 *__restrict_
   return reinterpret_cast<const float *>(p)[2 * i];
 }
 
-template <class F, int P, int NF, bool NOFLUX, int BY>
+template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
 __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
     const typename F::T *__restrict__ pfrac, const uint8_t *__restrict__ flag,
@@ -303,7 +334,6 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
     const int zchunk, const int tiles_x, const int tiles_y, const int nblocks) {
   using A = typename F::A;
   using C = typename F::C;
-  using R = RawCell<F, P>;
   const int b = blockIdx.x;
   const int per = nblocks >> 3, rem = nblocks & 7, xcd = b & 7;
   const int logical = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);
@@ -374,50 +404,32 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
       }
     }
     // cells next to a phase boundary: the full partition + weighted flux, one plane at a time
-    while (general) {
-      const int u = __builtin_ctz(general);
-      general &= general - 1;
-      const long base = plane * (z + u);
-      long nbi[6];
+    // (SPLIT: diffuse_general_kernel updates them from the per-sweep list instead)
+    if (!SPLIT) {
+      while (general) {
+        const int u = __builtin_ctz(general);
+        general &= general - 1;
+        const long base = plane * (z + u);
+        long nbi[6];
 #pragma unroll
-      for (int f = 0; f < 4; ++f) nbi[f] = base + on[f];
-      nbi[4] = oc + plane * zidx(z + u - 1);
-      nbi[5] = oc + plane * zidx(z + u + 1);
-      const long i = base + oc;
-      R r;
-      r.c = cin[i];
-#pragma unroll
-      for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
-      CellPF<A> me;
-      partition_cell<F, P>(r, dc, me);
-      A del = A(0);
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        R rn;
-        rn.c = cin[nbi[f]];
-#pragma unroll
-        for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
-        CellPF<A> nb;
-        partition_cell<F, P>(rn, dc, nb);
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-          del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
+        for (int f = 0; f < 4; ++f) nbi[f] = base + on[f];
+        nbi[4] = oc + plane * zidx(z + u - 1);
+        nbi[5] = oc + plane * zidx(z + u + 1);
+        const long i = base + oc;
+        cout[i] = full_update<F, P, NF>(cin, pfrac, i, nbi, dc);
       }
-      del *= dc.r;
-      cout[i] = conc_update<F>(r.c, del);
     }
   }
 }
 
 // Storage-order variant: one thread per cell, workgroups over consecutive cells (sequential HBM
 // streams); the six neighbours come from the caches.  Same arithmetic as diffuse_bulk_kernel.
-template <class F, int P, int NF, bool NOFLUX>
+template <class F, int P, int NF, bool NOFLUX, bool SPLIT>
 __global__ __launch_bounds__(256) void diffuse_linear_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
     const typename F::T *__restrict__ pfrac, const uint8_t *__restrict__ flag,
     const DiffConsts<typename F::A> dc, const int nx, const int ny, const int nz) {
   using A = typename F::A;
-  using R = RawCell<F, P>;
   const long plane = long(nx) * ny;
   const long i = long(blockIdx.x) * 256 + threadIdx.x;
   if (i >= plane * nz) return;
@@ -435,9 +447,8 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
   nbi[4] = i + plane * (zm - z);
   nbi[5] = i + plane * (zp - z);
   const uint8_t fl = flag[i];
-  const typename F::C cv = cin[i];
-  A del = A(0);
   if (fl != 0) {
+    const typename F::C cv = cin[i];
     A Dp = dc.D[0];
 #pragma unroll
     for (int q = 1; q < P; ++q)
@@ -446,30 +457,98 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
     A nv[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) nv[f] = A(conc_hi(cin[nbi[f]]));
+    A del = A(0);
 #pragma unroll
     for (int f = 0; f < NF; ++f) del += (nv[f] - c) * Dp;
-  } else {
-    R r;
-    r.c = cv;
-#pragma unroll
-    for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
-    CellPF<A> me;
-    partition_cell<F, P>(r, dc, me);
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      R rn;
-      rn.c = cin[nbi[f]];
-#pragma unroll
-      for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
-      CellPF<A> nb;
-      partition_cell<F, P>(rn, dc, nb);
-#pragma unroll
-      for (int p = 0; p < P; ++p)
-        del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
-    }
+    del *= dc.r;
+    cout[i] = conc_update<F>(cv, del);
+  } else if (!SPLIT) {
+    cout[i] = full_update<F, P, NF>(cin, pfrac, i, nbi, dc);
   }
-  del *= dc.r;
-  cout[i] = conc_update<F>(cv, del);
+}
+
+// Cells next to a phase boundary (flag 0), from the list built once per sweep: the full
+// partition + weighted flux, one thread per listed cell (grid-stride over the device-side count).
+template <class F, int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void diffuse_general_kernel(
+    const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
+    const typename F::T *__restrict__ pfrac, const int *__restrict__ list,
+    const int *__restrict__ count, const DiffConsts<typename F::A> dc, const int nx,
+    const int ny, const int nz) {
+  const int n = *count;
+  const long plane = long(nx) * ny;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+    const long i = list[k];
+    const int x = int(i % nx);
+    const long yz = i / nx;
+    const int y = int(yz % ny);
+    const int z = int(yz / ny);
+    long nbi[6];
+    nbi[0] = i - x + wrap(x - 1, nx);
+    nbi[1] = i - x + wrap(x + 1, nx);
+    nbi[2] = i + long(nx) * (wrap(y - 1, ny) - y);
+    nbi[3] = i + long(nx) * (wrap(y + 1, ny) - y);
+    const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+    const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+    nbi[4] = i + plane * (zm - z);
+    nbi[5] = i + plane * (zp - z);
+    cout[i] = full_update<F, P, NF>(cin, pfrac, i, nbi, dc);
+  }
+}
+
+// ---- list of the cells that need the full formula (flag 0), in storage order -------------------
+// count: per 256-cell block; scan: exclusive offsets (one workgroup); scatter: ranks by ballot.
+__global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__restrict__ flag,
+                                                            long ncell, int *__restrict__ bcount) {
+  __shared__ int wsum[4];
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  const bool g = i < ncell && flag[i] == 0;
+  const unsigned long long m = __ballot(g);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// exclusive prefix sum of bcount[0..nb) in place; bcount[nb] = total.  One workgroup of 1024.
+__global__ __launch_bounds__(1024) void general_scan_kernel(int *__restrict__ bcount, int nb) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int lo = min(nb, t * per), hi = min(nb, lo + per);
+  int s = 0;
+  for (int k = lo; k < hi; ++k) s += bcount[k];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;  // exclusive start of this thread's range
+  for (int k = lo; k < hi; ++k) {
+    const int c = bcount[k];
+    bcount[k] = run;
+    run += c;
+  }
+  if (t == 1023) bcount[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__restrict__ flag,
+                                                              long ncell,
+                                                              const int *__restrict__ boff,
+                                                              int *__restrict__ list) {
+  __shared__ int wsum[4];
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  const bool g = i < ncell && flag[i] == 0;
+  const unsigned long long m = __ballot(g);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = __popcll(m);
+  __syncthreads();
+  int base = boff[blockIdx.x];
+  for (int w = 0; w < wave; ++w) base += wsum[w];
+  if (g) list[base + __popcll(m & ((1ull << lane) - 1ull))] = int(i);
 }
 
 // explicit instantiations used by the host dispatch (pf_api.hip)
@@ -484,18 +563,24 @@ PF_INST_CLASSIFY(double, 2)
 PF_INST_CLASSIFY(double, 3)
 PF_INST_CLASSIFY(float, 2)
 PF_INST_CLASSIFY(float, 3)
+#define PF_INST_BULK(F, P, NF, NOF, BY, SPLIT)                                                  \
+  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, BY, SPLIT>(                       \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
+      int, int, int, int);
 #define PF_INST_DIFF_ONE(F, P, NF, NOF)                                                           \
-  template __global__ void diffuse_linear_kernel<F, P, NF, NOF>(                                  \
+  template __global__ void diffuse_linear_kernel<F, P, NF, NOF, false>(                           \
       const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int); \
-  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, 4>(                                 \
-      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
-      int, int, int, int);                                                                      \
-  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, 8>(                                 \
-      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
-      int, int, int, int);                                                                      \
-  template __global__ void diffuse_bulk_kernel<F, P, NF, NOF, 16>(                                \
-      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int, \
-      int, int, int, int);                                                                      \
+  template __global__ void diffuse_linear_kernel<F, P, NF, NOF, true>(                            \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>, int, int, int); \
+  template __global__ void diffuse_general_kernel<F, P, NF, NOF>(                                 \
+      const F::C *, F::C *, const F::T *, const int *, const int *, const DiffConsts<F::A>, int,  \
+      int, int);                                                                                \
+  PF_INST_BULK(F, P, NF, NOF, 4, false)                                                         \
+  PF_INST_BULK(F, P, NF, NOF, 8, false)                                                         \
+  PF_INST_BULK(F, P, NF, NOF, 16, false)                                                        \
+  PF_INST_BULK(F, P, NF, NOF, 4, true)                                                          \
+  PF_INST_BULK(F, P, NF, NOF, 8, true)                                                          \
+  PF_INST_BULK(F, P, NF, NOF, 16, true)                                                         \
   template __global__ void diffuse_kernel<F, P, NF, NOF, kDiffTY>(                                \
       const F::C *, F::C *, const F::T *, const DiffConsts<F::A>, int, int, int, int, int, int, \
       int);

@@ -28,7 +28,8 @@ __global__ void diffuse_kernel(const typename F::C *__restrict__ cin,
 constexpr int kMarchBY = 8;     // default rows of the bulk diffusion tile (64 x kMarchBY lanes)
 constexpr int kDiffUnroll = 4;  // planes per iteration of a column (loads issued together)
 
-template <class F, int P, int NF, bool NOFLUX>
+// SPLIT: cells with flag 0 are left to diffuse_general_kernel (list built per sweep)
+template <class F, int P, int NF, bool NOFLUX, bool SPLIT>
 __global__ void diffuse_linear_kernel(const typename F::C *__restrict__ cin,
                                       typename F::C *__restrict__ cout,
                                       const typename F::T *__restrict__ pfrac,
@@ -36,12 +37,26 @@ __global__ void diffuse_linear_kernel(const typename F::C *__restrict__ cin,
                                       const DiffConsts<typename F::A> dc, const int nx,
                                       const int ny, const int nz);
 
+template <class F, int P, int NF, bool NOFLUX>
+__global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
+                                       typename F::C *__restrict__ cout,
+                                       const typename F::T *__restrict__ pfrac,
+                                       const int *__restrict__ list, const int *__restrict__ count,
+                                       const DiffConsts<typename F::A> dc, const int nx,
+                                       const int ny, const int nz);
+
+__global__ void general_count_kernel(const uint8_t *__restrict__ flag, long ncell,
+                                     int *__restrict__ bcount);
+__global__ void general_scan_kernel(int *__restrict__ bcount, int nb);
+__global__ void general_scatter_kernel(const uint8_t *__restrict__ flag, long ncell,
+                                       const int *__restrict__ boff, int *__restrict__ list);
+
 template <class T, int P, int NF, bool NOFLUX>
 __global__ void classify_kernel(const T *__restrict__ pfrac, uint8_t *__restrict__ flag,
                                 const int nx, const int ny, const int nz, const long ncell,
                                 const unsigned mask);
 
-template <class F, int P, int NF, bool NOFLUX, int BY>
+template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
 __global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,
                                     typename F::C *__restrict__ cout,
                                     const typename F::T *__restrict__ pfrac,

@@ -117,14 +117,18 @@ def test_full_steps_bitwise(tmp_path, shape, bc, K):
     assert_same(gs, os_, K, "full step")
 
 
-@pytest.mark.parametrize("kernel,rows", [("bulk", "0"), ("bulk", "4"), ("bulk", "8"), ("bulk", "16"),
-                                         ("lds", "-1")])
-def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows):
-    """Every diffusion kernel (storage order, column-marching with 4/8/16-row tiles, LDS-tiled
-    full formula) against the oracle on a seeded microstructure (bulk and interface cells)."""
-    banner(f"diffusion variant {kernel} rows={rows}")
+@pytest.mark.parametrize("kernel,rows,split", [
+    ("bulk", "0", "1"), ("bulk", "4", "1"), ("bulk", "8", "1"), ("bulk", "16", "1"),
+    ("bulk", "0", "0"), ("bulk", "4", "0"), ("bulk", "8", "0"), ("bulk", "16", "0"),
+    ("lds", "-1", "0")])
+def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, split):
+    """Every diffusion kernel (storage order, column-marching with 4/8/16-row tiles, each with the
+    full-formula cells inline or in their own listed pass; LDS-tiled full formula) against the
+    oracle on a seeded microstructure (bulk and interface cells)."""
+    banner(f"diffusion variant {kernel} rows={rows} split={split}")
     monkeypatch.setenv("PFAMD_DIFF_KERNEL", kernel)
     monkeypatch.setenv("PFAMD_DIFF_BY", rows)
+    monkeypatch.setenv("PFAMD_DIFF_SPLIT", split)
     m, p, st = _setup_state(tmp_path, nx=70, ny=36, nz=22, grains=6, nuclei=5)
     gs, os_ = run_pair((70, 36, 22), 4, 2, 1, 0, oracle.ORC_FP64, st, p,
                        lambda g, q: g.step(2), lambda o, q: o.step(2))
