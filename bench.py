@@ -57,9 +57,10 @@ def setup_fp64(m, p):
 def alg_bytes_per_substep(mixed: bool, nphase: int) -> int:
     """Compulsory bytes of one carbon diffusion sub-step per cell: read value_phase (P values) and
     the stored concentration, write the concentration (FP64: 8-byte values; mixed: 4-byte phase
-    fractions and an 8-byte {value, Kahan term} concentration)."""
-    b_phi = 4 if mixed else 8
-    return nphase * b_phi + 8 + 8
+    fractions and a 6-byte concentration = FP32 value + 16-bit Kahan compensation code)."""
+    if mixed:
+        return nphase * 4 + 6 + 6
+    return nphase * 8 + 8 + 8
 
 
 def diffusion_kernel_name(mixed: bool, nphase: int, ncell: int) -> str:

@@ -17,7 +17,8 @@
 // layout.  The reference itself cannot be compiled here (Blitz++ is not installed).
 //
 // Three number forms share the code: FP64 (the reference), MIXED (FP32 storage and arithmetic,
-// FP64 for the clamp/prune decision and renormalisation, Kahan-compensated concentration update)
+// FP64 for the clamp/prune decision and renormalisation, Kahan-compensated concentration update
+// with the compensation held as a 16-bit fixed-point fraction of an ulp)
 // and DUAL (FP64 with forward-mode derivatives, the reference gradient for the adjoint).
 
 #include "pf_oracle.h"
@@ -91,6 +92,18 @@ template <> inline float as<float, double>(const double &x) { return static_cast
 template <> inline double as<double, Dual>(const Dual &x) { return x.v; }
 
 using std::sqrt;
+
+// Mixed form: the Kahan compensation is kept as a 16-bit fixed-point multiple of 2^-15 ulp of the
+// stored value (comp = q * 2^(e - 39), value = m * 2^e with 0.5 <= |m| < 1, q saturating at the
+// int16 range); this returns the compensation that representation holds.
+inline float quantize_comp(float comp, float value) {
+  int e = 0;
+  (void)std::frexp(value, &e);
+  float q = std::rint(std::ldexp(comp, 39 - e));
+  q = std::fmin(std::fmax(q, -32768.f), 32767.f);
+  return std::ldexp(q, e - 39);
+}
+template <class T> inline T quantize_comp(T comp, T) { return comp; }
 
 template <class S, class A, class D, bool K> struct Form {
   using Store = S;  // phi, value_phase, concentrations
@@ -403,7 +416,7 @@ template <class F> struct Domain : DomainBase {
             if (F::kahan) {
               const S y = S(del) - comp[s][c];
               const S t = conc[s][c] + y;
-              comp[s][c] = (t - conc[s][c]) - y;
+              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t);
               conc[s][c] = t;
             } else {
               conc[s][c] += S(del);

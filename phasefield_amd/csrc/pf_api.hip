@@ -419,7 +419,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   h->dims = dims;
   h->NF = 2 * dims;
   h->tsz = d.precision == PF_MIXED ? 4 : 8;
-  h->csz = 8;  // double, or float2 {value, compensation}
+  h->csz = d.precision == PF_MIXED ? 6 : 8;  // double, or float value + int16 compensation code
   if (const char *e = std::getenv("PFAMD_DIFF_ZCHUNK")) h->diff_zchunk_override = std::atoi(e);
   if (const char *e = std::getenv("PFAMD_DIFF_KERNEL"))
     h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
@@ -534,11 +534,15 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
     HIPCHK(h, hipMemcpyAsync(h->phi[0], v.data(), NK * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->pfrac, ph.data(), ph.size() * 4, hipMemcpyHostToDevice, h->stream));
     for (int sol = 0; sol < h->d.nsolute; ++sol) {
-      std::vector<float2> c(N);
-      for (long n = 0; n < N; ++n)
-        c[n] = make_float2(float(s->conc[sol * N + n]),
-                           s->conc_comp ? float(s->conc_comp[sol * N + n]) : 0.f);
-      HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), N * 8, hipMemcpyHostToDevice, h->stream));
+      // [value][compensation code] per buffer
+      std::vector<float> c(size_t(N) + (size_t(N) + 1) / 2);
+      int16_t *q = comp_codes(c.data(), N);
+      for (long n = 0; n < N; ++n) {
+        c[n] = float(s->conc[sol * N + n]);
+        q[n] = s->conc_comp ? comp_encode(float(s->conc_comp[sol * N + n]), c[n]) : int16_t(0);
+      }
+      HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), N * h->csz, hipMemcpyHostToDevice,
+                               h->stream));
       HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -595,11 +599,13 @@ int pf_download_state(pf_handle *h, pf_host_state *s) {
     for (long n = 0; n < N; ++n)
       for (int p = 0; p < P; ++p) phase[size_t(n) * 3 + p] = ph[size_t(n) * P + p];
     for (int sol = 0; sol < h->d.nsolute; ++sol) {
-      std::vector<float2> c(N);
-      HIPCHK(h, hipMemcpy(c.data(), h->conc[sol][h->ccur[sol]], N * 8, hipMemcpyDeviceToHost));
+      std::vector<float> c(size_t(N) + (size_t(N) + 1) / 2);
+      HIPCHK(h, hipMemcpy(c.data(), h->conc[sol][h->ccur[sol]], N * h->csz,
+                          hipMemcpyDeviceToHost));
+      const int16_t *q = comp_codes(c.data(), N);
       for (long n = 0; n < N; ++n) {
-        conc[sol * N + n] = c[n].x;
-        comp[sol * N + n] = c[n].y;
+        conc[sol * N + n] = c[n];
+        comp[sol * N + n] = comp_decode(q[n], c[n]);
       }
     }
   } else {
@@ -1027,7 +1033,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
       dst->ccur[s] = 0;
       hipLaunchKernelGGL(convert_conc_f64_to_mixed, dim3(blocks_for(long(N))), dim3(256), 0,
                          dst->stream, static_cast<const double *>(src->conc[s][src->ccur[s]]),
-                         static_cast<float2 *>(dst->conc[s][0]), long(N));
+                         static_cast<float *>(dst->conc[s][0]), long(N));
     }
     HIPCHK(dst, hipGetLastError());
   } else {

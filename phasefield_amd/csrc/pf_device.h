@@ -5,7 +5,8 @@
 //   slot phi    T      [N][K]    double (FP64 form) or float (mixed form), double-buffered
 //   phase frac  T      [N][P]    value_phase, P = 2 or 3 phases carried
 //   conc        C      [S][N]    per solute, double-buffered (Jacobi sub-steps):
-//                                C = double (FP64) or float2 {value, Kahan compensation} (mixed)
+//                                FP64: double [N];  mixed: float value [N] followed by the int16
+//                                Kahan compensation code [N] in the same buffer (6 B per cell)
 // A slot id is a 16-bit grain code: bits 15..14 = phase (1..3), bits 13..0 = grain serial;
 // 0 = empty slot.  The host maps the reference's int32 grain ids (100000+n, 200000+g) to codes
 // and back; the step only ever compares codes for equality, so results do not depend on it.
@@ -30,18 +31,45 @@ struct FormF64 {
 struct FormMixed {
   using T = float;
   using A = float;
-  using C = float2;    // {c, Kahan compensation}; effective value c - comp
+  using C = float;     // stored value; its Kahan compensation code follows the N values
   static constexpr bool kMixed = true;
 };
 
-template <class C> __device__ inline double conc_value(C c);
-template <> __device__ inline double conc_value<double>(double c) { return c; }
-template <> __device__ inline double conc_value<float2>(float2 c) {
-  return double(c.x) - double(c.y);
+template <class C> __host__ __device__ inline C conc_hi(C c) { return c; }
+
+// Mixed-form Kahan compensation, stored as a 16-bit fixed-point multiple of the value's ulp:
+// comp = q * 2^(e - 39) where c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. steps of 2^-15 ulp(c)
+// (ulp(c) = 2^(e - 24)).  |comp| <= ulp/2 after a Kahan update, so |q| <= 2^14 with 2x headroom;
+// codes saturate at the int16 range.  The effective concentration is c - comp.
+constexpr int kCompShift = 39;
+__host__ __device__ inline int comp_exp(float c) {
+  int e = 0;
+  (void)frexpf(c, &e);
+  return e;
 }
-template <class C> __device__ inline auto conc_hi(C c);
-template <> __device__ inline auto conc_hi<double>(double c) { return c; }
-template <> __device__ inline auto conc_hi<float2>(float2 c) { return c.x; }
+__host__ __device__ inline float comp_decode(int16_t q, float c) {
+  return ldexpf(float(q), comp_exp(c) - kCompShift);
+}
+__host__ __device__ inline int16_t comp_encode(float comp, float c) {
+  float v = rintf(ldexpf(comp, kCompShift - comp_exp(c)));
+  v = fminf(fmaxf(v, -32768.f), 32767.f);
+  return int16_t(v);
+}
+// effective stored concentration of cell i of a buffer of n cells (mixed: value - compensation)
+template <class F> __device__ inline double conc_effective(const typename F::C *buf, long i, long n);
+template <> __device__ inline double conc_effective<FormF64>(const double *buf, long i, long) {
+  return buf[i];
+}
+// compensation codes of a mixed concentration buffer of n cells
+__host__ __device__ inline const int16_t *comp_codes(const float *buf, long n) {
+  return reinterpret_cast<const int16_t *>(buf + n);
+}
+__host__ __device__ inline int16_t *comp_codes(float *buf, long n) {
+  return reinterpret_cast<int16_t *>(buf + n);
+}
+template <> __device__ inline double conc_effective<FormMixed>(const float *buf, long i, long n) {
+  return double(buf[i]) - double(comp_decode(comp_codes(buf, n)[i], buf[i]));
+}
 
 // periodic wrap of i+d for |d| <= n (Get_neighbor, phase_field.h:58-68, 0-based)
 __device__ inline int wrap(int i, int n) {

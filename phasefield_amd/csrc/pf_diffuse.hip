@@ -70,22 +70,35 @@ template <class A> __device__ inline A face_weight(A prod) {
   return w;
 }
 
-template <class F> __device__ inline typename F::C conc_update(typename F::C c, typename F::A del);
-template <> __device__ inline double conc_update<FormF64>(double c, double del) { return c + del; }
-template <> __device__ inline float2 conc_update<FormMixed>(float2 c, float del) {
-  // Kahan-compensated increment: y = del - comp; t = c + y; comp = (t - c) - y; c = t
-  const float y = del - c.y;
-  const float t = c.x + y;
-  return make_float2(t, (t - c.x) - y);
+// total_conc += del for cell i (solver.cpp:366) from its stored value c read from `in` (n cells
+// per buffer).  Mixed form: Kahan-compensated, y = del - comp; t = c + y; comp = (t - c) - y;
+// c = t, with the compensation code stored after the n values of the buffer.
+template <class F>
+__device__ inline void conc_update(const typename F::C *__restrict__ in,
+                                   typename F::C *__restrict__ out, long i, long n,
+                                   typename F::C c, typename F::A del);
+template <>
+__device__ inline void conc_update<FormF64>(const double *__restrict__, double *__restrict__ out,
+                                            long i, long, double c, double del) {
+  out[i] = c + del;
+}
+template <>
+__device__ inline void conc_update<FormMixed>(const float *__restrict__ in, float *__restrict__ out,
+                                              long i, long n, float c, float del) {
+  const float comp = comp_decode(comp_codes(in, n)[i], c);
+  const float y = del - comp;
+  const float t = c + y;
+  out[i] = t;
+  comp_codes(out, n)[i] = comp_encode((t - c) - y, t);
 }
 
 // The reference's sub-step for one cell from its own and its stencil's stored concentration and
 // value_phase: partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365), update.
 template <class F, int P, int NF>
-__device__ inline typename F::C full_update(const typename F::C *__restrict__ cin,
-                                            const typename F::T *__restrict__ pfrac, long i,
-                                            const long (&nbi)[6],
-                                            const DiffConsts<typename F::A> &dc) {
+__device__ inline void full_update(const typename F::C *__restrict__ cin,
+                                   typename F::C *__restrict__ cout,
+                                   const typename F::T *__restrict__ pfrac, long i, long n,
+                                   const long (&nbi)[6], const DiffConsts<typename F::A> &dc) {
   using A = typename F::A;
   RawCell<F, P> r;
   r.c = cin[i];
@@ -107,7 +120,7 @@ __device__ inline typename F::C full_update(const typename F::C *__restrict__ ci
       del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
   }
   del *= dc.r;
-  return conc_update<F>(r.c, del);
+  conc_update<F>(cin, cout, i, n, r.c, del);
 }
 
 template <class F, int P, int NF, bool NOFLUX, int TY>
@@ -244,7 +257,7 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
         for (int p = 0; p < P; ++p) wmz[p] = wpz[p];
       }
       del *= dc.r;
-      cout[col + plane * z] = conc_update<F>(ccur, del);
+      conc_update<F>(cin, cout, col + plane * z, plane * nz, ccur, del);
     }
     prv = cur;
     cur = nxt;
@@ -320,10 +333,7 @@ __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfr
 // lanes index them with 32-bit column offsets.
 template <class F> __device__ inline typename F::A load_hi(const typename F::C *__restrict__ p,
                                                           int i) {
-  return typename F::A(conc_hi(p[i]));
-}
-template <> __device__ inline float load_hi<FormMixed>(const float2 *__restrict__ p, int i) {
-  return reinterpret_cast<const float *>(p)[2 * i];
+  return typename F::A(p[i]);
 }
 
 template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
@@ -397,7 +407,7 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
             del += (p - c) * Dp;
           }
           del *= dc.r;
-          cout[plane * (z + u) + oc] = conc_update<F>(cc[u], del);
+          conc_update<F>(cin, cout, plane * (z + u) + oc, plane * nz, cc[u], del);
         } else {
           general |= 1u << u;
         }
@@ -416,7 +426,7 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
         nbi[4] = oc + plane * zidx(z + u - 1);
         nbi[5] = oc + plane * zidx(z + u + 1);
         const long i = base + oc;
-        cout[i] = full_update<F, P, NF>(cin, pfrac, i, nbi, dc);
+        full_update<F, P, NF>(cin, cout, pfrac, i, plane * nz, nbi, dc);
       }
     }
   }
@@ -461,9 +471,9 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
 #pragma unroll
     for (int f = 0; f < NF; ++f) del += (nv[f] - c) * Dp;
     del *= dc.r;
-    cout[i] = conc_update<F>(cv, del);
+    conc_update<F>(cin, cout, i, plane * nz, cv, del);
   } else if (!SPLIT) {
-    cout[i] = full_update<F, P, NF>(cin, pfrac, i, nbi, dc);
+    full_update<F, P, NF>(cin, cout, pfrac, i, plane * nz, nbi, dc);
   }
 }
 
@@ -492,7 +502,7 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
     const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
     nbi[4] = i + plane * (zm - z);
     nbi[5] = i + plane * (zp - z);
-    cout[i] = full_update<F, P, NF>(cin, pfrac, i, nbi, dc);
+    full_update<F, P, NF>(cin, cout, pfrac, i, plane * nz, nbi, dc);
   }
 }
 

@@ -107,7 +107,7 @@ __global__ void solute_noise_kernel(double mean, double sd, uint64_t seed_key, l
                                     double *__restrict__ c);
 __global__ void convert_f64_to_f32(const double *__restrict__ src, float *__restrict__ dst,
                                    long n);
-__global__ void convert_conc_f64_to_mixed(const double *__restrict__ src, float2 *__restrict__ dst,
+__global__ void convert_conc_f64_to_mixed(const double *__restrict__ src, float *__restrict__ dst,
                                           long n);
 
 }  // namespace pfamd

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void reduce_stage1(const uint16_t *__restrict_
   for (long c = beg + threadIdx.x; c < end; c += 256) {
     bool bad = false;
     for (int s = 0; s < ns; ++s) {
-      const double v = conc_value((s == 0 ? c0 : c1)[c]);
+      const double v = conc_effective<F>(s == 0 ? c0 : c1, c, ncell);
       r.mass[s] += v;
       r.cmax[s] = fmax(r.cmax[s], v);
       r.cmin[s] = fmin(r.cmin[s], v);
@@ -372,9 +372,12 @@ __global__ __launch_bounds__(256) void convert_f64_to_f32(const double *__restri
   if (i < n) dst[i] = float(src[i]);
 }
 __global__ __launch_bounds__(256) void convert_conc_f64_to_mixed(const double *__restrict__ src,
-                                                                 float2 *__restrict__ dst, long n) {
+                                                                 float *__restrict__ dst, long n) {
   const long i = long(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) dst[i] = make_float2(float(src[i]), 0.f);
+  if (i < n) {
+    dst[i] = float(src[i]);
+    comp_codes(dst, n)[i] = 0;
+  }
 }
 
 // explicit instantiations used by the host dispatch (pf_api.hip)
