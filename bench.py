@@ -63,12 +63,29 @@ def alg_bytes_per_substep(mixed: bool, nphase: int) -> int:
     return nphase * 8 + 8 + 8
 
 
+def measured_traffic(workload: str) -> dict:
+    """HBM bytes per carbon sub-step measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a
+    plain copy of known size), from the `TRAFFIC {...}` line of the summaries in profiles/."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return {}
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(d, name)) as f:
+            for line in f:
+                if line.startswith("TRAFFIC "):
+                    rec = json.loads(line[len("TRAFFIC "):])
+                    if rec.get("workload") == workload:
+                        return rec
+    return {}
+
+
 def diffusion_kernel_name(mixed: bool, nphase: int, ncell: int) -> str:
     """The sub-step kernel the library launches for this domain (periodic 3D, default choice)."""
     form = "FormMixed" if mixed else "FormF64"
-    if ncell >= 1 << 26:
-        return f"diffuse_bulk_kernel<{form},{nphase},6,false,4>"
-    return f"diffuse_linear_kernel<{form},{nphase},6,false>"
+    return (f"diffuse_bulk_vec_kernel<{form},{nphase},6,false,4,{4 if mixed else 2}> + "
+            f"diffuse_general_kernel<{form},{nphase},6,false> (one carbon sub-step)")
 
 
 def timed_run(dom, model, steps: int, warmup: int):
@@ -319,6 +336,7 @@ def main() -> int:
         return 0
 
     res = {}
+    traffic = measured_traffic(args.workload)
     for name, dom, model in (("mixed", mixed, m_mixed), ("fp64", g64, m_fp64)):
         if name == "fp64" and args.no_fp64:
             continue
@@ -333,7 +351,8 @@ def main() -> int:
                      "diff_launches": dn, "pf_us": pms / pn * 1e3, "stats": st,
                      "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                                  "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                  "traffic": traffic.get("FormMixed" if name == "mixed" else "FormF64"),
                                   "kernel": diffusion_kernel_name(name == "mixed", d.nphase, N),
                                   "bytes_per_cell": bpc, "cells_per_launch": N,
                                   "launch_us": round(launch_us, 2)}}
@@ -348,7 +367,8 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Voronoi + nucleation)",
         "config": {"workload": f"{args.workload}: {d.nx}^3 Fe-C{'-Mn' if d.nsolute > 1 else ''}, "
                                f"{wl['grains']} Voronoi ferrite grains + {wl['nuclei']} austenite "
-                               f"nuclei (r=4), periodic, K={d.depth}, P={d.nphase}, "
+                               f"nuclei (r=4), {'no-flux z' if wl['bc_z'] == 'noflux' else 'periodic'}, "
+                               f"K={d.depth}, P={d.nphase}, "
                                f"{p0.n_sub[0]} carbon sub-steps"
                                f"{' + 1 Mn update' if d.nsolute > 1 else ''} per step",
                    "form": "mixed (FP32 + Kahan, FP64 prune/renormalise)",

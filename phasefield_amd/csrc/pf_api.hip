@@ -64,8 +64,9 @@ struct pf_handle {
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
   bool diff_split = true;    // full-formula cells from a per-sweep list in their own kernel
+  bool diff_vec = true;      // bulk-only sub-step with 16-byte runs per lane (when nx allows)
   int *glist = nullptr;      // indices of the cells with flag 0, storage order
-  int *gblock = nullptr;     // per-256-cell-block offsets into glist; [nblocks] = list length
+  int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
 };
 
@@ -152,6 +153,8 @@ void update_bulk_mask(pf_handle *h) {
 
 // ---- launch helpers -----------------------------------------------------------------------------
 inline unsigned blocks_for(long n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+// workgroups of the full-formula list build (kListChunk * 256 cells each)
+inline unsigned list_blocks(long n) { return blocks_for(n, 256 * kListChunk); }
 
 // One event pair on the handle's stream around `count` back-to-back launches of one kind (events
 // between every launch would add their own cost to what they measure).
@@ -214,10 +217,11 @@ struct DiffGeom {
   int tiles_x, tiles_y, zchunk, nblocks;
 };
 
-// rows: tile height; target: workgroups to aim for (z is split into chunks until reached)
-DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk) {
+// rows: tile height; target: workgroups to aim for (z is split into chunks until reached);
+// cols: x cells per tile row
+DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk, int cols = 64) {
   DiffGeom g;
-  g.tiles_x = (h->d.nx + 63) / 64;
+  g.tiles_x = (h->d.nx + cols - 1) / cols;
   g.tiles_y = (h->d.ny + rows - 1) / rows;
   const int nz = h->d.nz;
   int zc;
@@ -253,11 +257,12 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
       hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(nb), dim3(256), 0, h->stream, pf,
                          h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask);
       if (h->diff_split) {
-        hipLaunchKernelGGL(general_count_kernel, dim3(nb), dim3(256), 0, h->stream, h->flag, h->N,
+        const unsigned nl = list_blocks(h->N);
+        hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, h->N,
                            h->gblock);
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock,
-                           int(nb));
-        hipLaunchKernelGGL(general_scatter_kernel, dim3(nb), dim3(256), 0, h->stream, h->flag,
+                           int(nl));
+        hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag,
                            h->N, h->gblock, h->glist);
       }
       h->flag_valid = true;
@@ -286,13 +291,21 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
       else
         go(std::integral_constant<int, 4>(), split);
     };
-    if (h->diff_split) {
+    constexpr int V = F::kMixed ? 4 : 2;  // 16-byte runs per lane
+    if (h->diff_split && h->diff_vec && h->d.nx % V == 0 && (h->diff_by < 0 || h->diff_by == 4)) {
+      const DiffGeom g = diff_geom(h, 4, 131072 / 4, 8, 64 * V);
+      hipLaunchKernelGGL((diffuse_bulk_vec_kernel<F, P, NF, NOF, 4, V>), dim3(g.nblocks),
+                         dim3(64, 4), 0, h->stream, cin, cout, h->flag, dc, h->d.nx, h->d.ny,
+                         h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
+    } else if (h->diff_split) {
       go_split(std::true_type());
+    }
+    if (h->diff_split) {
       // the listed cells: a grid-stride pass sized for the whole chip, bounded by the list length
       const unsigned ng = std::min<unsigned>(nb, 2048);
       hipLaunchKernelGGL((diffuse_general_kernel<F, P, NF, NOF>), dim3(ng), dim3(256), 0,
-                         h->stream, cin, cout, pf, h->glist, h->gblock + nb, dc, h->d.nx, h->d.ny,
-                         h->d.nz);
+                         h->stream, cin, cout, pf, h->glist, h->gblock + list_blocks(h->N), dc,
+                         h->d.nx, h->d.ny, h->d.nz);
     } else {
       go_split(std::false_type());
     }
@@ -425,6 +438,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
   if (const char *e = std::getenv("PFAMD_DIFF_BY")) h->diff_by = std::atoi(e);
   if (const char *e = std::getenv("PFAMD_DIFF_SPLIT")) h->diff_split = std::atoi(e) != 0;
+  if (const char *e = std::getenv("PFAMD_DIFF_VEC")) h->diff_vec = std::atoi(e) != 0;
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {
     if (e == hipSuccess) return PF_OK;
@@ -446,8 +460,8 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     return rc;
   if ((rc = chk(hipMalloc(&h->flag, N), "alloc bulk flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
-  if ((rc = chk(hipMalloc(&h->gblock, (blocks_for(h->N) + 1) * sizeof(int)), "alloc list offsets")) !=
-      PF_OK)
+  if ((rc = chk(hipMalloc(&h->gblock, (list_blocks(h->N) + 1) * sizeof(int)),
+                "alloc list offsets")) != PF_OK)
     return rc;
   h->red_blocks = int(std::min<long>(1024, std::max<long>(1, (h->N + 4095) / 4096)));
   if ((rc = chk(hipMalloc(&h->red, sizeof(RedPartial) * h->red_blocks), "alloc partials")) != PF_OK)

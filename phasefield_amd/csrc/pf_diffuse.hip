@@ -432,6 +432,142 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
   }
 }
 
+// Vector form of the bulk-only sub-step (the full-formula cells are left to the listed pass): a
+// lane owns V consecutive x cells of one row (16-byte loads of its own and its y rows), takes the
+// x neighbours at the ends of its run with two scalar loads, and walks a z-chunk U planes at a
+// time.  No branch on the flag: cells with flag 0 get a value that diffuse_general_kernel
+// overwrites afterwards (same stream), so every store is a full vector.  Needs nx % V == 0.
+template <class A, int V> struct alignas(sizeof(A) * V) VecA {
+  A v[V];
+};
+template <int V> struct alignas(V) VecFlag {
+  uint8_t f[V];
+};
+template <int V> struct alignas(2 * V) VecCode {
+  int16_t q[V];
+};
+
+template <class F, int V>
+__device__ inline void vec_update(const typename F::C *__restrict__ in, typename F::C *__restrict__ out,
+                                  long i, long n, const VecA<typename F::A, V> &c,
+                                  const typename F::A (&del)[V]);
+template <>
+__device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double *__restrict__ out,
+                                              long i, long, const VecA<double, 2> &c,
+                                              const double (&del)[2]) {
+  VecA<double, 2> o;
+  o.v[0] = c.v[0] + del[0];
+  o.v[1] = c.v[1] + del[1];
+  *reinterpret_cast<VecA<double, 2> *>(out + i) = o;
+}
+template <>
+__device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, float *__restrict__ out,
+                                                long i, long n, const VecA<float, 4> &c,
+                                                const float (&del)[4]) {
+  const VecCode<4> qi = *reinterpret_cast<const VecCode<4> *>(comp_codes(in, n) + i);
+  VecA<float, 4> o;
+  VecCode<4> qo;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const float comp = comp_decode(qi.q[v], c.v[v]);
+    const float y = del[v] - comp;
+    const float t = c.v[v] + y;
+    o.v[v] = t;
+    qo.q[v] = comp_encode((t - c.v[v]) - y, t);
+  }
+  *reinterpret_cast<VecA<float, 4> *>(out + i) = o;
+  *reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i) = qo;
+}
+
+template <class F, int P, int NF, bool NOFLUX, int BY, int V>
+__global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
+    const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
+    const uint8_t *__restrict__ flag, const DiffConsts<typename F::A> dc, const int nx,
+    const int ny, const int nz, const int zchunk, const int tiles_x, const int tiles_y,
+    const int nblocks) {
+  using A = typename F::A;
+  using VA = VecA<A, V>;
+  const int b = blockIdx.x;
+  const int per = nblocks >> 3, rem = nblocks & 7, xcd = b & 7;
+  const int logical = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);
+  const int tx_t = logical % tiles_x;
+  const int t2 = logical / tiles_x;
+  const int ty_t = t2 % tiles_y;
+  const int tz = t2 / tiles_y;
+  const int x0 = (tx_t * 64 + int(threadIdx.x)) * V, y = ty_t * BY + threadIdx.y, z0 = tz * zchunk;
+  if (x0 >= nx || y >= ny) return;  // no barrier in this kernel
+  const int z1 = min(z0 + zchunk, nz);
+  const long plane = long(nx) * ny;
+  const long ncell = plane * nz;
+  const int oc = x0 + nx * y;  // column offset of the run within a plane
+  const int oxm = wrap(x0 - 1, nx) + nx * y, oxp = wrap(x0 + V, nx) + nx * y;
+  const int oym = x0 + nx * wrap(y - 1, ny), oyp = x0 + nx * wrap(y + 1, ny);
+  auto zidx = [&](int z) -> int {
+    if (NOFLUX) return z < 0 ? 0 : (z >= nz ? nz - 1 : z);
+    return z < 0 ? z + nz : (z >= nz ? z - nz : z);
+  };
+  auto ldv = [&](const typename F::C *pl, int o) -> VA {
+    return *reinterpret_cast<const VA *>(pl + o);
+  };
+  constexpr int U = 4;
+  for (int z = z0; z < z1; z += U) {
+    const int nv = min(U, z1 - z);  // planes of this batch (wave-uniform)
+    VA cc[U], ym[U], yp[U];
+    A xm[U], xp[U];
+    VecFlag<V> fl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long base = plane * min(z + u, z1 - 1);
+      const typename F::C *__restrict__ pl = cin + base;
+      fl[u] = *reinterpret_cast<const VecFlag<V> *>(flag + base + oc);
+      cc[u] = ldv(pl, oc);
+      xm[u] = A(pl[oxm]);
+      xp[u] = A(pl[oxp]);
+      if (NF >= 4) {
+        ym[u] = ldv(pl, oym);
+        yp[u] = ldv(pl, oyp);
+      }
+    }
+    VA zlo, zhi;
+    if (NF == 6) {
+      zlo = ldv(cin + plane * zidx(z - 1), oc);
+      zhi = ldv(cin + plane * zidx(z + nv), oc);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (u < nv) {
+        A del[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int fv = fl[u].f[v];
+          A Dp = dc.D[0];
+#pragma unroll
+          for (int q = 1; q < P; ++q)
+            if (fv == q + 1) Dp = dc.D[q];
+          const A c = cc[u].v[v];
+          const A l = v == 0 ? xm[u] : cc[u].v[v > 0 ? v - 1 : 0];
+          const A r = v == V - 1 ? xp[u] : cc[u].v[v + 1 < V ? v + 1 : v];
+          A d = A(0);
+          d += (l - c) * Dp;
+          d += (r - c) * Dp;
+          if (NF >= 4) {
+            d += (ym[u].v[v] - c) * Dp;
+            d += (yp[u].v[v] - c) * Dp;
+          }
+          if (NF == 6) {
+            const A m = u == 0 ? zlo.v[v] : cc[u > 0 ? u - 1 : 0].v[v];
+            const A p = u + 1 < nv ? cc[u + 1 < U ? u + 1 : u].v[v] : zhi.v[v];
+            d += (m - c) * Dp;
+            d += (p - c) * Dp;
+          }
+          del[v] = d * dc.r;
+        }
+        vec_update<F, V>(cin, cout, plane * (z + u) + oc, ncell, cc[u], del);
+      }
+    }
+  }
+}
+
 // Storage-order variant: one thread per cell, workgroups over consecutive cells (sequential HBM
 // streams); the six neighbours come from the caches.  Same arithmetic as diffuse_bulk_kernel.
 template <class F, int P, int NF, bool NOFLUX, bool SPLIT>
@@ -507,15 +643,19 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
 }
 
 // ---- list of the cells that need the full formula (flag 0), in storage order -------------------
-// count: per 256-cell block; scan: exclusive offsets (one workgroup); scatter: ranks by ballot.
+// Blocks of kListChunk*256 cells.  count: cells per block; scan: exclusive offsets (one
+// workgroup); scatter: storage-order ranks from wave ballots, chunk by chunk.
 __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__restrict__ flag,
                                                             long ncell, int *__restrict__ bcount) {
   __shared__ int wsum[4];
-  const long i = long(blockIdx.x) * 256 + threadIdx.x;
-  const bool g = i < ncell && flag[i] == 0;
-  const unsigned long long m = __ballot(g);
+  const long base = long(blockIdx.x) * (256L * kListChunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wsum[wave] = __popcll(m);
+  int n = 0;
+  for (int k = 0; k < kListChunk; ++k) {
+    const long i = base + long(k) * 256 + threadIdx.x;
+    n += __popcll(__ballot(i < ncell && flag[i] == 0));
+  }
+  if (lane == 0) wsum[wave] = n;
   __syncthreads();
   if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
@@ -549,16 +689,21 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
                                                               long ncell,
                                                               const int *__restrict__ boff,
                                                               int *__restrict__ list) {
-  __shared__ int wsum[4];
-  const long i = long(blockIdx.x) * 256 + threadIdx.x;
-  const bool g = i < ncell && flag[i] == 0;
-  const unsigned long long m = __ballot(g);
+  __shared__ int wsum[2][4];
+  const long base = long(blockIdx.x) * (256L * kListChunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wsum[wave] = __popcll(m);
-  __syncthreads();
-  int base = boff[blockIdx.x];
-  for (int w = 0; w < wave; ++w) base += wsum[w];
-  if (g) list[base + __popcll(m & ((1ull << lane) - 1ull))] = int(i);
+  int run = boff[blockIdx.x];
+  for (int k = 0; k < kListChunk; ++k) {
+    const long i = base + long(k) * 256 + threadIdx.x;
+    const bool g = i < ncell && flag[i] == 0;
+    const unsigned long long m = __ballot(g);
+    if (lane == 0) wsum[k & 1][wave] = __popcll(m);
+    __syncthreads();  // wsum[k & 1] is rewritten two chunks later, after the next barrier
+    int w0 = run;
+    for (int w = 0; w < wave; ++w) w0 += wsum[k & 1][w];
+    if (g) list[w0 + __popcll(m & ((1ull << lane) - 1ull))] = int(i);
+    run += wsum[k & 1][0] + wsum[k & 1][1] + wsum[k & 1][2] + wsum[k & 1][3];
+  }
 }
 
 // explicit instantiations used by the host dispatch (pf_api.hip)
@@ -594,6 +739,20 @@ PF_INST_CLASSIFY(float, 3)
   template __global__ void diffuse_kernel<F, P, NF, NOF, kDiffTY>(                                \
       const F::C *, F::C *, const F::T *, const DiffConsts<F::A>, int, int, int, int, int, int, \
       int);
+#define PF_INST_VEC(F, P, V)                                                                       \
+  template __global__ void diffuse_bulk_vec_kernel<F, P, 4, false, 4, V>(                         \
+      const F::C *, F::C *, const uint8_t *, const DiffConsts<F::A>, int, int, int, int, int, int, \
+      int);                                                                                      \
+  template __global__ void diffuse_bulk_vec_kernel<F, P, 6, false, 4, V>(                         \
+      const F::C *, F::C *, const uint8_t *, const DiffConsts<F::A>, int, int, int, int, int, int, \
+      int);                                                                                      \
+  template __global__ void diffuse_bulk_vec_kernel<F, P, 6, true, 4, V>(                          \
+      const F::C *, F::C *, const uint8_t *, const DiffConsts<F::A>, int, int, int, int, int, int, \
+      int);
+PF_INST_VEC(FormF64, 2, 2)
+PF_INST_VEC(FormF64, 3, 2)
+PF_INST_VEC(FormMixed, 2, 4)
+PF_INST_VEC(FormMixed, 3, 4)
 #define PF_INST_DIFF(F, P)               \
   PF_INST_DIFF_ONE(F, P, 4, false)       \
   PF_INST_DIFF_ONE(F, P, 6, false)       \

@@ -37,6 +37,16 @@ __global__ void diffuse_linear_kernel(const typename F::C *__restrict__ cin,
                                       const DiffConsts<typename F::A> dc, const int nx,
                                       const int ny, const int nz);
 
+// bulk-only sub-step with V consecutive x cells per lane (nx % V == 0); full-formula cells are
+// overwritten by diffuse_general_kernel afterwards
+template <class F, int P, int NF, bool NOFLUX, int BY, int V>
+__global__ void diffuse_bulk_vec_kernel(const typename F::C *__restrict__ cin,
+                                        typename F::C *__restrict__ cout,
+                                        const uint8_t *__restrict__ flag,
+                                        const DiffConsts<typename F::A> dc, const int nx,
+                                        const int ny, const int nz, const int zchunk,
+                                        const int tiles_x, const int tiles_y, const int nblocks);
+
 template <class F, int P, int NF, bool NOFLUX>
 __global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
                                        typename F::C *__restrict__ cout,
@@ -45,6 +55,7 @@ __global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
                                        const DiffConsts<typename F::A> dc, const int nx,
                                        const int ny, const int nz);
 
+constexpr int kListChunk = 16;  // 256-cell chunks per block of the full-formula list build
 __global__ void general_count_kernel(const uint8_t *__restrict__ flag, long ncell,
                                      int *__restrict__ bcount);
 __global__ void general_scan_kernel(int *__restrict__ bcount, int nb);

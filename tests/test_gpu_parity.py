@@ -117,22 +117,29 @@ def test_full_steps_bitwise(tmp_path, shape, bc, K):
     assert_same(gs, os_, K, "full step")
 
 
-@pytest.mark.parametrize("kernel,rows,split", [
-    ("bulk", "0", "1"), ("bulk", "4", "1"), ("bulk", "8", "1"), ("bulk", "16", "1"),
-    ("bulk", "0", "0"), ("bulk", "4", "0"), ("bulk", "8", "0"), ("bulk", "16", "0"),
-    ("lds", "-1", "0")])
-def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, split):
-    """Every diffusion kernel (storage order, column-marching with 4/8/16-row tiles, each with the
-    full-formula cells inline or in their own listed pass; LDS-tiled full formula) against the
-    oracle on a seeded microstructure (bulk and interface cells)."""
-    banner(f"diffusion variant {kernel} rows={rows} split={split}")
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("kernel,rows,split,vec", [
+    ("bulk", "-1", "1", "1"),
+    ("bulk", "0", "1", "0"), ("bulk", "4", "1", "0"), ("bulk", "8", "1", "0"),
+    ("bulk", "16", "1", "0"),
+    ("bulk", "0", "0", "0"), ("bulk", "4", "0", "0"), ("bulk", "8", "0", "0"),
+    ("bulk", "16", "0", "0"),
+    ("lds", "-1", "0", "0")])
+def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, split, vec, form):
+    """Every diffusion kernel (16-byte-run bulk kernel; storage order and column-marching with
+    4/8/16-row tiles, each with the full-formula cells inline or in their own listed pass;
+    LDS-tiled full formula) against the oracle on a seeded microstructure (bulk and interface
+    cells), in both forms."""
+    banner(f"diffusion variant {kernel} rows={rows} split={split} vec={vec} form={form}")
     monkeypatch.setenv("PFAMD_DIFF_KERNEL", kernel)
     monkeypatch.setenv("PFAMD_DIFF_BY", rows)
     monkeypatch.setenv("PFAMD_DIFF_SPLIT", split)
-    m, p, st = _setup_state(tmp_path, nx=70, ny=36, nz=22, grains=6, nuclei=5)
-    gs, os_ = run_pair((70, 36, 22), 4, 2, 1, 0, oracle.ORC_FP64, st, p,
+    monkeypatch.setenv("PFAMD_DIFF_VEC", vec)
+    shape = (72, 36, 22)  # nx a multiple of 4 and not of 64: partial vector tiles
+    m, p, st = _setup_state(tmp_path, nx=shape[0], ny=shape[1], nz=shape[2], grains=6, nuclei=5)
+    gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p,
                        lambda g, q: g.step(2), lambda o, q: o.step(2))
-    assert_same(gs, os_, 4, f"{kernel}/{rows}")
+    assert_same(gs, os_, 4, f"{kernel}/{rows}/{split}/{vec}")
 
 
 def test_bulk_reduction_needs_identity_partition(tmp_path):
@@ -270,3 +277,28 @@ def test_compat_shim_matches_oracle(base_params, rng):
     o.set_state(ids, val, phase, conc)
     o.diffuse_factor(0, nloop, dt / nloop / (dx * dx))
     assert np.array_equal(c, o.get_state()["conc"])
+
+
+@pytest.mark.slow
+def test_fec_3d_256_steps_bitwise(tmp_path):
+    """The 256^3 Fe-C parity workload at full size (128 grains, 200 nuclei, K = 4): the device-built
+    seeded state is handed to the oracle, both run 5 complete steps (sweep + 200 sub-steps each),
+    and active-set maps, phi, value_phase and concentrations must agree bit for bit."""
+    from phasefield_amd import Model, configs
+    banner("fec_3d_256: 5 full steps vs oracle")
+    m = Model(configs.write_inputs(str(tmp_path), "fec_3d_256"))
+    p = m.update_parameter()
+    d = m.desc()
+    g = Domain.from_desc(d)
+    g.set_params(p)
+    g.construct(m.voronoi_seeds(), m.get_real("reduction"), m.get_int("num_init_iterations"),
+                [m.get_real(f"init_conc{i}") for i in range(3)])
+    g.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"),
+               (m.get_real("nucleus_conc"), -1.0), m.get_int("nucleation_seed"))
+    s0 = g.download()
+    o = oracle.OracleDomain(d.nx, d.ny, d.nz, d.depth, 1)
+    o.set_params(oracle.as_orc_params(p))
+    o.set_state(s0["index"], s0["value"], s0["phase"], s0["conc"])
+    g.step(5)
+    o.step(5)
+    assert_same(g.download(), o.get_state(), d.depth, "256^3 x 5 steps")
