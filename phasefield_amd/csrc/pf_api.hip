@@ -67,6 +67,9 @@ struct pf_handle {
   bool diff_vec = true;      // bulk-only sub-step with 16-byte runs per lane (when nx allows)
   int *glist = nullptr;      // indices of the cells with flag 0, storage order
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
+  bool sweep_split = true;   // copy cells classified and written apart from the listed sweep
+  uint8_t *sflag = nullptr;  // per cell: 1 = copied by the last sweep
+  bool sweep_hist = false;   // sflag describes the last sweep of the current buffers
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
 };
 
@@ -187,11 +190,34 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
   using T = typename F::T;
   using C = typename F::C;
   const int a = h->cur, b = 1 - h->cur;
-  hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, ISO>), dim3(blocks_for(h->N, kSweepBS)),
-                     dim3(kSweepBS), 0, h->stream, h->id[a], static_cast<const T *>(h->phi[a]), h->id[b],
-                     static_cast<T *>(h->phi[b]), static_cast<T *>(h->pfrac),
-                     static_cast<const C *>(h->conc[0][h->ccur[0]]), cs, h->d.nx, h->d.ny, h->d.nz,
-                     h->N);
+  const T *phi1 = static_cast<const T *>(h->phi[a]);
+  T *phi2 = static_cast<T *>(h->phi[b]);
+  const C *c0 = static_cast<const C *>(h->conc[0][h->ccur[0]]);
+  if (!ISO && h->sweep_split) {
+    // copy cells first (written, or skipped when the destination already holds them), then the
+    // full update over the list of the others
+    const unsigned nb = blocks_for(h->N), nl = list_blocks(h->N);
+    hipLaunchKernelGGL((sweep_classify_kernel<F, K, NF, NOF>), dim3(nb), dim3(256), 0, h->stream,
+                       h->id[a], phi1, h->id[b], phi2, h->sflag, h->sweep_hist ? 1 : 0,
+                       h->p.cutoff, h->d.nx, h->d.ny, h->d.nz, h->N);
+    hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag, h->N,
+                       h->gblock);
+    hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock, int(nl));
+    hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag, h->N,
+                       h->gblock, h->glist);
+    // grid-stride over the list: enough workgroups to fill every CU at the LDS-limited occupancy
+    const unsigned ng = std::min<unsigned>(blocks_for(h->N, kSweepBS), 4096);
+    hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true>), dim3(ng), dim3(kSweepBS),
+                       0, h->stream, h->id[a], phi1, h->id[b], phi2, static_cast<T *>(h->pfrac),
+                       c0, cs, h->d.nx, h->d.ny, h->d.nz, h->N, h->glist, h->gblock + nl);
+    h->sweep_hist = true;
+  } else {
+    hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, ISO, false>),
+                       dim3(blocks_for(h->N, kSweepBS)), dim3(kSweepBS), 0, h->stream, h->id[a],
+                       phi1, h->id[b], phi2, static_cast<T *>(h->pfrac), c0, cs, h->d.nx, h->d.ny,
+                       h->d.nz, h->N, nullptr, nullptr);
+    h->sweep_hist = false;
+  }
   h->cur = b;
   h->flag_valid = false;
   return hipGetLastError();
@@ -376,6 +402,7 @@ void free_all(pf_handle *h) {
   if (h->pfrac) hipFree(h->pfrac);
   if (h->flag) hipFree(h->flag);
   if (h->glist) hipFree(h->glist);
+  if (h->sflag) hipFree(h->sflag);
   if (h->gblock) hipFree(h->gblock);
   if (h->red) hipFree(h->red);
   for (int k = 0; k < 2; ++k)
@@ -439,6 +466,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if (const char *e = std::getenv("PFAMD_DIFF_BY")) h->diff_by = std::atoi(e);
   if (const char *e = std::getenv("PFAMD_DIFF_SPLIT")) h->diff_split = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_DIFF_VEC")) h->diff_vec = std::atoi(e) != 0;
+  if (const char *e = std::getenv("PFAMD_SWEEP_SPLIT")) h->sweep_split = std::atoi(e) != 0;
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {
     if (e == hipSuccess) return PF_OK;
@@ -460,6 +488,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     return rc;
   if ((rc = chk(hipMalloc(&h->flag, N), "alloc bulk flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->sflag, N), "alloc sweep copy flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->gblock, (list_blocks(h->N) + 1) * sizeof(int)),
                 "alloc list offsets")) != PF_OK)
     return rc;
@@ -575,6 +604,7 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
   h->ccur[0] = h->ccur[1] = 0;
   h->have_state = true;
   h->flag_valid = false;
+  h->sweep_hist = false;
   return PF_OK;
 }
 
@@ -857,6 +887,7 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
   }
   h->have_state = true;
   h->flag_valid = false;
+  h->sweep_hist = false;
   return PF_OK;
 }
 
@@ -920,6 +951,7 @@ int pf_setup_construct(pf_handle *h, const double *seeds, int G, double reductio
   for (int g = 1; g <= G; ++g) h->code_to_id[2].push_back(200000 + g);
   h->have_state = true;
   h->flag_valid = false;
+  h->sweep_hist = false;
   return PF_OK;
 }
 
@@ -1002,6 +1034,7 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
   h->code_to_id[1].assign(1, 0);
   for (int q = 1; q <= n_nuclei; ++q) h->code_to_id[1].push_back(100000 + q);
   h->flag_valid = false;
+  h->sweep_hist = false;
   return PF_OK;
 }
 
@@ -1065,6 +1098,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
   for (int ph = 0; ph < 4; ++ph) dst->code_to_id[ph] = src->code_to_id[ph];
   dst->have_state = true;
   dst->flag_valid = false;
+  dst->sweep_hist = false;
   return PF_OK;
 }
 
@@ -1208,7 +1242,10 @@ int handle_diffuse_io(pf_handle *h, const double *cin, double *cout) {
   if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("diffuse: ") + hipGetErrorString(e));
   return PF_OK;
 }
-void handle_state_changed(pf_handle *h) { h->flag_valid = false; }
+void handle_state_changed(pf_handle *h) {
+  h->flag_valid = false;
+  h->sweep_hist = false;
+}
 PFConsts<double> handle_pf_consts(const pf_handle *h) { return make_pf_consts<double>(h); }
 DiffConsts<double> handle_diff_consts(const pf_handle *h, int solute, double factor) {
   return make_diff_consts<double>(h->p, solute, factor);

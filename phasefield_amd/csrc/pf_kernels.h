@@ -8,14 +8,24 @@ namespace pfamd {
 constexpr int kDiffTY = 8;     // rows of the diffusion tile (64 x kDiffTY threads)
 constexpr int kSweepBS = 128;  // threads per phase-field sweep block (one cell each)
 
-template <class F, int K, int P, int NF, bool NOFLUX, bool ISO>
+template <class F, int K, int P, int NF, bool NOFLUX, bool ISO, bool LIST>
 __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const typename F::T *__restrict__ phi1,
                                 uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
                                 typename F::T *__restrict__ pfrac,
                                 const typename F::C *__restrict__ conc0,
                                 const PFConsts<typename F::A> cs, const int nx, const int ny,
-                                const int nz, const long ncell);
+                                const int nz, const long ncell, const int *__restrict__ list,
+                                const int *__restrict__ count);
+
+// copy cells of the sweep written directly, the others flagged 0 for the listed sweep
+template <class F, int K, int NF, bool NOFLUX>
+__global__ void sweep_classify_kernel(const uint16_t *__restrict__ id1,
+                                      const typename F::T *__restrict__ phi1,
+                                      uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
+                                      uint8_t *__restrict__ sflag, const int history,
+                                      const double cutoff, const int nx, const int ny,
+                                      const int nz, const long ncell);
 
 template <class F, int P, int NF, bool NOFLUX, int TY>
 __global__ void diffuse_kernel(const typename F::C *__restrict__ cin,

@@ -36,13 +36,14 @@ __device__ inline A pair_pick(const A (&tab)[4][4], int pa, int pb) {
   return r;
 }
 
-template <class F, int K, int P, int NF, bool NOFLUX, bool ISO>
+// LIST: the cells come from list[0 .. *count) (grid-stride); otherwise one cell per thread.
+template <class F, int K, int P, int NF, bool NOFLUX, bool ISO, bool LIST>
 __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     const uint16_t *__restrict__ id1, const typename F::T *__restrict__ phi1,
     uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
     typename F::T *__restrict__ pfrac, const typename F::C *__restrict__ conc0,
     const PFConsts<typename F::A> cs, const int nx, const int ny, const int nz,
-    const long ncell) {
+    const long ncell, const int *__restrict__ list, const int *__restrict__ count) {
   using T = typename F::T;
   using A = typename F::A;
   __shared__ uint32_t sLg[kStencilMax][kSweepBS];
@@ -50,8 +51,8 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   __shared__ T sV2[kStencilMax][kSweepBS];
   __shared__ double sDel[kStencilMax][kSweepBS];
   const int t = threadIdx.x;
-  const long cell = long(blockIdx.x) * kSweepBS + t;
-  if (cell >= ncell) return;  // no block-wide barrier below
+  // no block-wide barrier below: threads run their cells independently
+  auto body = [&](const long cell) {
   const int x = int(cell % nx);
   const long yz = cell / nx;
   const int y = int(yz % ny);
@@ -244,15 +245,116 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   for (int s = 0; s < K; ++s) phi2[cell * K + s] = T(out[s] * inv);
 #pragma unroll
   for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
+  };
+  if (LIST) {
+    const int n = *count;
+    for (int k = blockIdx.x * kSweepBS + t; k < n; k += gridDim.x * kSweepBS) body(list[k]);
+  } else {
+    const long cell = long(blockIdx.x) * kSweepBS + t;
+    if (cell < ncell) body(cell);
+  }
 }
 
-#define PF_INST_SWEEP(F, K, P, NF, NOF, ISO)                                    \
-  template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO>(              \
+// Copy cells of the sweep (solver.cpp:207-216: a cell whose active-set union holds a single entry
+// keeps its slots, value_phase untouched).  The union has one entry exactly when at most one own
+// slot is above the cutoff and every neighbour id before its list's first zero is that grain (or,
+// with no own entry, all those ids are one grain).  Copy cells are written here; the others get
+// flag 0 for the listed full sweep.  sflag holds 1 for a cell copied by the previous sweep: its
+// destination row already equals its source row then (the buffers swap every sweep and a copy
+// makes them equal), so the write is skipped when `history` is set.
+template <class F, int K, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void sweep_classify_kernel(
+    const uint16_t *__restrict__ id1, const typename F::T *__restrict__ phi1,
+    uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2, uint8_t *__restrict__ sflag,
+    const int history, const double cutoff, const int nx, const int ny, const int nz,
+    const long ncell) {
+  using T = typename F::T;
+  const long cell = long(blockIdx.x) * 256 + threadIdx.x;
+  if (cell >= ncell) return;
+  uint32_t g0[K];
+  T v0[K];
+  load_ids<K>(id1, cell, g0);
+#pragma unroll
+  for (int s = 0; s < K; ++s) v0[s] = phi1[cell * K + s];
+  int n0 = 0;
+  uint32_t G = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (double(v0[s]) > cutoff) {
+      ++n0;
+      G = g0[s];
+    }
+  bool copy = n0 <= 1;
+  bool any = n0 == 1;
+  if (copy) {
+    const int x = int(cell % nx);
+    const long yz = cell / nx;
+    const int y = int(yz % ny);
+    const int z = int(yz / ny);
+    const long plane = long(nx) * ny;
+    long nb[6];
+    nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
+    nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;
+    nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
+    nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
+    const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+    const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+    nb[4] = x + long(nx) * y + plane * zm;
+    nb[5] = x + long(nx) * y + plane * zp;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      uint32_t gn[K];
+      load_ids<K>(id1, nb[f], gn);
+      bool live = true;
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        live = live && gn[s] != 0;
+        if (live) {
+          if (!any) {
+            G = gn[s];
+            any = true;
+          } else if (gn[s] != G) {
+            copy = false;
+          }
+        }
+      }
+    }
+    copy = copy && any;
+  }
+  if (copy) {
+    if (!(history && sflag[cell] != 0)) {
+      using V = typename IdRowT<K>::V;
+      reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
+#pragma unroll
+      for (int s = 0; s < K; ++s) phi2[cell * K + s] = v0[s];
+    }
+    sflag[cell] = 1;
+  } else {
+    sflag[cell] = 0;
+  }
+}
+
+#define PF_INST_SWEEP(F, K, P, NF, NOF, ISO, LIST)                              \
+  template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO, LIST>(        \
       const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *, \
-      const PFConsts<F::A>, int, int, int, long);
-#define PF_INST_SWEEP_ALL(F, K, P, ISO)   \
-  PF_INST_SWEEP(F, K, P, 4, false, ISO)   \
-  PF_INST_SWEEP(F, K, P, 6, false, ISO)   \
-  PF_INST_SWEEP(F, K, P, 6, true, ISO)
+      const PFConsts<F::A>, int, int, int, long, const int *, const int *);
+#define PF_INST_SWEEP_ALL(F, K, P, ISO)          \
+  PF_INST_SWEEP(F, K, P, 4, false, ISO, false)   \
+  PF_INST_SWEEP(F, K, P, 6, false, ISO, false)   \
+  PF_INST_SWEEP(F, K, P, 6, true, ISO, false)
+#define PF_INST_SWEEP_LIST(F, K, P)             \
+  PF_INST_SWEEP(F, K, P, 4, false, false, true)  \
+  PF_INST_SWEEP(F, K, P, 6, false, false, true)  \
+  PF_INST_SWEEP(F, K, P, 6, true, false, true)
+#define PF_INST_SWEEP_CLASSIFY(F, K)                                                        \
+  template __global__ void sweep_classify_kernel<F, K, 4, false>(                          \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, int, int, \
+      int, long);                                                                          \
+  template __global__ void sweep_classify_kernel<F, K, 6, false>(                          \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, int, int, \
+      int, long);                                                                          \
+  template __global__ void sweep_classify_kernel<F, K, 6, true>(                           \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, int, int, \
+      int, long);
 
 }  // namespace pfamd

@@ -104,11 +104,14 @@ def _setup_state(tmp_path, **over):
     return m, p, (s["index"], s["value"], s["phase"], s["conc"])
 
 
+@pytest.mark.parametrize("sweep_split", ["1", "0"])
 @pytest.mark.parametrize("shape,bc,K", [((70, 20, 18), "periodic", 4), ((48, 40, 1), "periodic", 6),
                                         ((40, 36, 24), "noflux", 4)])
-def test_full_steps_bitwise(tmp_path, shape, bc, K):
-    """Several complete Update() steps (sweep + 200 sub-steps) from a seeded microstructure."""
-    banner(f"full steps {shape} {bc} K={K}")
+def test_full_steps_bitwise(tmp_path, monkeypatch, shape, bc, K, sweep_split):
+    """Several complete Update() steps (sweep + 200 sub-steps) from a seeded microstructure, with
+    the sweep's copy cells classified apart (and their unchanged rows not rewritten) or not."""
+    banner(f"full steps {shape} {bc} K={K} sweep_split={sweep_split}")
+    monkeypatch.setenv("PFAMD_SWEEP_SPLIT", sweep_split)
     nx, ny, nz = shape
     m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=6, nuclei=4, depth=K, bc_z=bc)
     bcn = 1 if bc == "noflux" else 0
@@ -302,3 +305,28 @@ def test_fec_3d_256_steps_bitwise(tmp_path):
     g.step(5)
     o.step(5)
     assert_same(g.download(), o.get_state(), d.depth, "256^3 x 5 steps")
+
+
+def test_sweep_history_across_restore(tmp_path):
+    """The sweep skips rewriting copy cells whose destination row already holds them; a restored
+    checkpoint (and an upload) must drop that knowledge: 2 + restore + 2 steps equal 4 steps."""
+    banner("sweep history across restore")
+    m, p, st = _setup_state(tmp_path, nx=64, ny=24, nz=20, grains=5, nuclei=4)
+    pp = oracle.as_orc_params(p)
+    from phasefield_amd._capi import PfParams
+    a = Domain(64, 24, 20, depth=4, nphase=2)
+    a.set_params(PfParams.from_buffer_copy(pp))
+    a.upload(*st)
+    a.step(2)
+    blob = a.checkpoint()
+    a.step(3)
+    a.restore(blob)
+    a.step(2)
+    b = Domain(64, 24, 20, depth=4, nphase=2)
+    b.set_params(PfParams.from_buffer_copy(pp))
+    b.upload(*st)
+    b.step(4)
+    assert_same(a.download(), b.download(), 4, "restore")
+    a.upload(*st)  # upload after history: starts over
+    a.step(4)
+    assert_same(a.download(), b.download(), 4, "re-upload")
