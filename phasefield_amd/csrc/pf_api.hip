@@ -1070,10 +1070,12 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
   HIPCHK(dst, hipMemcpyAsync(dst->id[0], src->id[src->cur], N * dst->KI * 2,
                              hipMemcpyDeviceToDevice, dst->stream));
   if (is_mixed(dst)) {
-    hipLaunchKernelGGL(convert_f64_to_f32, dim3(blocks_for(long(N) * a.depth)), dim3(256), 0,
+    hipLaunchKernelGGL(convert_f64_to_f32, dim3(std::min(blocks_for(long(N) * a.depth), 65536u)),
+                       dim3(256), 0,
                        dst->stream, static_cast<const double *>(src->phi[src->cur]),
                        static_cast<float *>(dst->phi[0]), long(N) * a.depth);
-    hipLaunchKernelGGL(convert_f64_to_f32, dim3(blocks_for(long(N) * a.nphase)), dim3(256), 0,
+    hipLaunchKernelGGL(convert_f64_to_f32, dim3(std::min(blocks_for(long(N) * a.nphase), 65536u)),
+                       dim3(256), 0,
                        dst->stream, static_cast<const double *>(src->pfrac),
                        static_cast<float *>(dst->pfrac), long(N) * a.nphase);
     for (int s = 0; s < a.nsolute; ++s) {

@@ -366,10 +366,11 @@ __global__ __launch_bounds__(256) void solute_noise_kernel(double mean, double s
 }
 
 // FP64 domain -> mixed domain (round each stored value; compensation starts at zero)
+// grid-stride: n reaches N*K elements (2^32 at 1024^3, K = 4), beyond one launch's work-items
 __global__ __launch_bounds__(256) void convert_f64_to_f32(const double *__restrict__ src,
                                                           float *__restrict__ dst, long n) {
-  const long i = long(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) dst[i] = float(src[i]);
+  for (long i = long(blockIdx.x) * 256 + threadIdx.x; i < n; i += long(gridDim.x) * 256)
+    dst[i] = float(src[i]);
 }
 __global__ __launch_bounds__(256) void convert_conc_f64_to_mixed(const double *__restrict__ src,
                                                                  float *__restrict__ dst, long n) {
