@@ -101,6 +101,66 @@ __device__ inline void block_accumulate(double v, double *__restrict__ acc) {
 // read (the renormalisation of such an output has zero derivative, a copied cell passes it on to
 // the same unit vector, and the initial value_phase is not a gradient output), so those cells
 // skip it.
+// Full reverse of one cell's sub-step (any cell; the formula reduces to the bulk form on bulk
+// cells): writes lam_out[j], adds the value_phase adjoint of non-unit cells, returns the cell's
+// dL/dD(phase 1) term.
+template <int P, int NF, bool NOFLUX>
+__device__ inline double adj_cell(long j, const double *__restrict__ c,
+                                  const double *__restrict__ pf, const double *__restrict__ lam_in,
+                                  double *__restrict__ lam_out, double *__restrict__ lam_pf,
+                                  const DiffConsts<double> &dc, int nx, int ny, int nz) {
+  long nb[6];
+  nbrs(j, nx, ny, nz, NOFLUX, nb);
+  const double lj = lam_in[j];
+  double dD0 = 0.0;
+  double phj[P];
+  int ones = 0, zeros = 0;
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    phj[p] = pf[j * P + p];
+    ones += phj[p] == 1.0;
+    zeros += phj[p] == 0.0;
+  }
+  const bool unit = ones == 1 && zeros == P - 1;
+  PartD<P> me;
+  part_deriv<P>(c[j], phj, dc, me);
+  double lout = lj, lpf[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) lpf[p] = 0.0;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const long n = nb[f];
+    double phn[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) phn[p] = pf[n * P + p];
+    PartD<P> o;
+    part_deriv<P>(c[n], phn, dc, o);
+    const double ln = lam_in[n];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const double prod = phn[p] * phj[p];
+      if (prod > 0.0) {
+        const double w = sqrt(prod);
+        const double dcp = o.cp[p] - me.cp[p];
+        lout += dc.r * dc.D[p] * w * me.g[p] * (ln - lj);
+        if (!unit) {
+          const double dw = phn[p] / (2.0 * w);  // dw/dphi_p(j)
+          lpf[p] += dc.r * dc.D[p] * dw * dcp * (lj - ln);
+#pragma unroll
+          for (int q = 0; q < P; ++q) lpf[q] += dc.r * dc.D[p] * w * me.h[p][q] * (ln - lj);
+        }
+        if (p == 0) dD0 += dc.r * lj * w * dcp;
+      }
+    }
+  }
+  lam_out[j] = lout;
+  if (!unit)
+#pragma unroll
+    for (int p = 0; p < P; ++p) lam_pf[j * P + p] += lpf[p];
+  return dD0;
+}
+
+// One thread per cell; bulk cells take the reduced form.
 template <int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void adj_diffuse_kernel(
     const double *__restrict__ c, const double *__restrict__ pf, const uint8_t *__restrict__ flag,
@@ -109,11 +169,11 @@ __global__ __launch_bounds__(256) void adj_diffuse_kernel(
   const long j = long(blockIdx.x) * 256 + threadIdx.x;
   double dD0 = 0.0;
   if (j < ncell) {
-    long nb[6];
-    nbrs(j, nx, ny, nz, NOFLUX, nb);
-    const double lj = lam_in[j];
     const int fp = flag[j];
     if (fp != 0) {
+      long nb[6];
+      nbrs(j, nx, ny, nz, NOFLUX, nb);
+      const double lj = lam_in[j];
       double sl = 0.0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) sl += lam_in[nb[f]] - lj;
@@ -126,53 +186,34 @@ __global__ __launch_bounds__(256) void adj_diffuse_kernel(
         dD0 = dc.r * lj * sc;
       }
     } else {
-      double phj[P];
-      int ones = 0, zeros = 0;
-#pragma unroll
-      for (int p = 0; p < P; ++p) {
-        phj[p] = pf[j * P + p];
-        ones += phj[p] == 1.0;
-        zeros += phj[p] == 0.0;
-      }
-      const bool unit = ones == 1 && zeros == P - 1;
-      PartD<P> me;
-      part_deriv<P>(c[j], phj, dc, me);
-      double lout = lj, lpf[P];
-#pragma unroll
-      for (int p = 0; p < P; ++p) lpf[p] = 0.0;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const long n = nb[f];
-        double phn[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) phn[p] = pf[n * P + p];
-        PartD<P> o;
-        part_deriv<P>(c[n], phn, dc, o);
-        const double ln = lam_in[n];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-          const double prod = phn[p] * phj[p];
-          if (prod > 0.0) {
-            const double w = sqrt(prod);
-            const double dcp = o.cp[p] - me.cp[p];
-            lout += dc.r * dc.D[p] * w * me.g[p] * (ln - lj);
-            if (!unit) {
-              const double dw = phn[p] / (2.0 * w);  // dw/dphi_p(j)
-              lpf[p] += dc.r * dc.D[p] * dw * dcp * (lj - ln);
-#pragma unroll
-              for (int q = 0; q < P; ++q) lpf[q] += dc.r * dc.D[p] * w * me.h[p][q] * (ln - lj);
-            }
-            if (p == 0) dD0 += dc.r * lj * w * dcp;
-          }
-        }
-      }
-      lam_out[j] = lout;
-      if (!unit)
-#pragma unroll
-        for (int p = 0; p < P; ++p) lam_pf[j * P + p] += lpf[p];
+      dD0 = adj_cell<P, NF, NOFLUX>(j, c, pf, lam_in, lam_out, lam_pf, dc, nx, ny, nz);
     }
   }
   block_accumulate(dD0, accD);
+}
+
+// Split form: the listed cells (flag 0 after adj_list_flag_kernel: interface cells and bulk cells
+// of phase 1, whose terms carry dL/dD(phase 1)) in a grid-stride pass, after the bulk cells were
+// updated by the forward bulk kernel run on lambda (for a bulk cell of phase p the reverse
+// sub-step lam + r D_p sum_faces (lam_nb - lam) is that kernel's update with lam for c).
+template <int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void adj_general_kernel(
+    const double *__restrict__ c, const double *__restrict__ pf, const int *__restrict__ list,
+    const int *__restrict__ count, const double *__restrict__ lam_in, double *__restrict__ lam_out,
+    double *__restrict__ lam_pf, double *__restrict__ accD, const DiffConsts<double> dc, int nx,
+    int ny, int nz) {
+  const int n = *count;
+  double dD0 = 0.0;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256)
+    dD0 += adj_cell<P, NF, NOFLUX>(list[k], c, pf, lam_in, lam_out, lam_pf, dc, nx, ny, nz);
+  block_accumulate(dD0, accD);
+}
+
+// bulk flags of phase 1 -> 0, so those cells join the listed pass
+__global__ __launch_bounds__(256) void adj_list_flag_kernel(uint8_t *__restrict__ flag,
+                                                            long ncell) {
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i < ncell && flag[i] == 1) flag[i] = 0;
 }
 
 // Reverse of the phase-field sweep of one cell.  Inputs: the sweep's input state (ids, phi, pf,
@@ -488,6 +529,27 @@ struct Arena {
   }
 };
 
+// Workgroup layout of the bulk sub-step kernels (64 lanes x 4 rows, `cols` x cells per tile row,
+// z split into chunks of at least 8 planes until ~32768 workgroups), as the forward step uses.
+struct TileGeom {
+  int tiles_x, tiles_y, zchunk, nblocks;
+};
+TileGeom tile_geom(int nx, int ny, int nz, int nf, int cols) {
+  TileGeom g;
+  g.tiles_x = (nx + cols - 1) / cols;
+  g.tiles_y = (ny + 3) / 4;
+  const long tiles = long(g.tiles_x) * g.tiles_y;
+  int zc = nz;
+  if (nf == 6) {
+    const long nzc = std::max(1L, std::min<long>(nz, (32768 + tiles - 1) / tiles));
+    zc = int((nz + nzc - 1) / nzc);
+    zc = std::max(zc, std::min(nz, 8));
+  }
+  g.zchunk = zc;
+  g.nblocks = int(tiles * ((nz + zc - 1) / zc));
+  return g;
+}
+
 // One step recomputed in the reverse sweep: its input state, value_phase after its sweep with the
 // bulk flags of that value_phase, and the carbon field before every sub-step (csub + N*s,
 // s = 0..n_sub; s = n_sub is the step's output).
@@ -526,6 +588,16 @@ template <int K, int P, int NF, bool NOF> struct Adj {
     double *accD = ar.get<double>(gb), *accMS = ar.get<double>(size_t(gs) * 2);
     double *part = ar.get<double>(gb);
     double *target = sp->target_conc ? ar.get<double>(N) : nullptr;
+    // split reverse sub-step: bulk cells by the forward bulk kernel on lambda, listed cells apart
+    const char *env_split = std::getenv("PFAMD_ADJ_SPLIT");
+    const bool split = !(env_split && std::strcmp(env_split, "0") == 0);
+    const bool vec = v.nx % 2 == 0;
+    const unsigned nl = unsigned((N + 256L * kListChunk - 1) / (256L * kListChunk));
+    const unsigned ngen = std::min<unsigned>(gb, 2048);
+    int *alist = split ? ar.get<int>(N) : nullptr;
+    int *ablock = split ? ar.get<int>(nl + 1) : nullptr;
+    const TileGeom gvec = tile_geom(v.nx, v.ny, v.nz, NF, 128);
+    const TileGeom gsc = tile_geom(v.nx, v.ny, v.nz, NF, 64);
     std::vector<StepData> seg(CK);
     for (auto &sd : seg) sd.in = ar.state(N, KI, K, P);
     // Sub-step concentrations for a whole segment when they fit (every step is then recomputed
@@ -641,10 +713,34 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         const double g = 2.0 * (frac[qs] - sp->target_fraction[qs]) / double(N);
         hipLaunchKernelGGL(adj_seed_fraction_kernel, dim3(gb), dim3(256), 0, st, lam_pf, P, g, N);
       }
+      if (split) {
+        // the listed cells of this step: interface cells and bulk cells of phase 1
+        hipLaunchKernelGGL(adj_list_flag_kernel, dim3(gb), dim3(256), 0, st, sd.flag, N);
+        hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, st, sd.flag, N, ablock);
+        hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl));
+        hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, N, ablock,
+                           alist);
+      }
       for (int s = nsub - 1; s >= 0; --s) {
-        hipLaunchKernelGGL((adj_diffuse_kernel<P, NF, NOF>), dim3(gb), dim3(256), 0, st,
-                           sd.csub + size_t(N) * s, sd.pf_new, sd.flag, lam_c, lam_c2, lam_pf,
-                           accD, dc, v.nx, v.ny, v.nz, N);
+        if (split) {
+          if (vec)
+            hipLaunchKernelGGL((diffuse_bulk_vec_kernel<FormF64, P, NF, NOF, 4, 2>),
+                               dim3(gvec.nblocks), dim3(64, 4), 0, st, lam_c, lam_c2, sd.flag, dc,
+                               v.nx, v.ny, v.nz, gvec.zchunk, gvec.tiles_x, gvec.tiles_y,
+                               gvec.nblocks);
+          else
+            hipLaunchKernelGGL((diffuse_bulk_kernel<FormF64, P, NF, NOF, 4, true>),
+                               dim3(gsc.nblocks), dim3(64, 4), 0, st, lam_c, lam_c2, sd.pf_new,
+                               sd.flag, dc, v.nx, v.ny, v.nz, gsc.zchunk, gsc.tiles_x, gsc.tiles_y,
+                               gsc.nblocks);
+          hipLaunchKernelGGL((adj_general_kernel<P, NF, NOF>), dim3(ngen), dim3(256), 0, st,
+                             sd.csub + size_t(N) * s, sd.pf_new, alist, ablock + nl, lam_c, lam_c2,
+                             lam_pf, accD, dc, v.nx, v.ny, v.nz);
+        } else {
+          hipLaunchKernelGGL((adj_diffuse_kernel<P, NF, NOF>), dim3(gb), dim3(256), 0, st,
+                             sd.csub + size_t(N) * s, sd.pf_new, sd.flag, lam_c, lam_c2, lam_pf,
+                             accD, dc, v.nx, v.ny, v.nz, N);
+        }
         std::swap(lam_c, lam_c2);
       }
       hipMemsetAsync(lam_phi2, 0, n8 * K, st);

@@ -57,13 +57,17 @@ def _gpu_domain(shape, bc, K, P, pp):
     return g
 
 
-CASES = [((20, 16, 12), "periodic", 4, 2), ((32, 28, 1), "periodic", 6, 2),
-         ((18, 14, 10), "noflux", 4, 3)]
+CASES = [((20, 16, 12), "periodic", 4, 2, "1"), ((32, 28, 1), "periodic", 6, 2, "1"),
+         ((18, 14, 10), "noflux", 4, 3, "1"), ((21, 16, 10), "periodic", 4, 2, "1"),
+         ((20, 16, 12), "periodic", 4, 2, "0")]
 
 
-@pytest.mark.parametrize("shape,bc,K,P", CASES)
-def test_adjoint_matches_dual_numbers(tmp_path, shape, bc, K, P):
-    banner(f"adjoint vs dual {shape} {bc} K={K} P={P}")
+@pytest.mark.parametrize("shape,bc,K,P,split", CASES)
+def test_adjoint_matches_dual_numbers(tmp_path, monkeypatch, shape, bc, K, P, split):
+    """Reverse sub-steps split (bulk cells by the forward bulk kernel on lambda: 16-byte runs, or the
+    scalar column kernel when nx is odd; listed cells apart) and unsplit."""
+    banner(f"adjoint vs dual {shape} {bc} K={K} P={P} split={split}")
+    monkeypatch.setenv("PFAMD_ADJ_SPLIT", split)
     rng = np.random.default_rng(7)
     pp, st = _case(tmp_path, shape, bc, K, P)
     bcn = 1 if bc == "noflux" else 0
