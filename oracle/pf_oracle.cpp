@@ -18,7 +18,7 @@
 //
 // Three number forms share the code: FP64 (the reference), MIXED (FP32 storage and arithmetic,
 // FP64 for the clamp/prune decision and renormalisation, Kahan-compensated concentration update
-// with the compensation held as a 16-bit fixed-point fraction of an ulp)
+// with the compensation held as a 14-bit fixed-point fraction of an ulp)
 // and DUAL (FP64 with forward-mode derivatives, the reference gradient for the adjoint).
 
 #include "pf_oracle.h"
@@ -93,15 +93,15 @@ template <> inline double as<double, Dual>(const Dual &x) { return x.v; }
 
 using std::sqrt;
 
-// Mixed form: the Kahan compensation is kept as a 16-bit fixed-point multiple of 2^-15 ulp of the
-// stored value (comp = q * 2^(e - 39), value = m * 2^e with 0.5 <= |m| < 1, q saturating at the
-// int16 range); this returns the compensation that representation holds.
+// Mixed form: the Kahan compensation is kept as a 14-bit signed fixed-point multiple of 2^-13 ulp
+// of the stored value (comp = q * 2^(e - 37), value = m * 2^e with 0.5 <= |m| < 1, q saturating at
+// [-8192, 8191]); this returns the compensation that representation holds.
 inline float quantize_comp(float comp, float value) {
   int e = 0;
   (void)std::frexp(value, &e);
-  float q = std::rint(std::ldexp(comp, 39 - e));
-  q = std::fmin(std::fmax(q, -32768.f), 32767.f);
-  return std::ldexp(q, e - 39);
+  float q = std::rint(std::ldexp(comp, 37 - e));
+  q = std::fmin(std::fmax(q, -8192.f), 8191.f);
+  return std::ldexp(q, e - 37);
 }
 template <class T> inline T quantize_comp(T comp, T) { return comp; }
 

@@ -696,7 +696,8 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       const DevState cur = on_handle();
       hipMemcpyAsync(sd.pf_new, cur.pf, n8 * P, hipMemcpyDeviceToDevice, st);
       hipLaunchKernelGGL((classify_kernel<double, P, NF, NOF>), dim3(gb), dim3(256), 0, st,
-                         sd.pf_new, sd.flag, v.nx, v.ny, v.nz, N, v.bulk_mask);
+                         sd.pf_new, sd.flag, v.nx, v.ny, v.nz, N, v.bulk_mask, nullptr,
+                         nullptr);
       hipMemcpyAsync(sd.csub, cur.c, n8, hipMemcpyDeviceToDevice, st);
       for (int s = 0; s < nsub && r == PF_OK; ++s)
         r = handle_diffuse_io(h, sd.csub + size_t(N) * s, sd.csub + size_t(N) * (s + 1));

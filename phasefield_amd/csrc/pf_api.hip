@@ -280,8 +280,13 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
   } else {
     const unsigned nb = blocks_for(h->N);
     if (!h->flag_valid || h->flag_nf != NF) {
+      int16_t *codes[2] = {nullptr, nullptr};
+      if (F::kMixed)
+        for (int q = 0; q < h->d.nsolute; ++q)
+          codes[q] = comp_codes(static_cast<float *>(h->conc[q][h->ccur[q]]), h->N);
       hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(nb), dim3(256), 0, h->stream, pf,
-                         h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask);
+                         h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask, codes[0],
+                         codes[1]);
       if (h->diff_split) {
         const unsigned nl = list_blocks(h->N);
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, h->N,
@@ -582,7 +587,7 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
       int16_t *q = comp_codes(c.data(), N);
       for (long n = 0; n < N; ++n) {
         c[n] = float(s->conc[sol * N + n]);
-        q[n] = s->conc_comp ? comp_encode(float(s->conc_comp[sol * N + n]), c[n]) : int16_t(0);
+        q[n] = s->conc_comp ? comp_encode(float(s->conc_comp[sol * N + n]), c[n], 0) : int16_t(0);
       }
       HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), N * h->csz, hipMemcpyHostToDevice,
                                h->stream));

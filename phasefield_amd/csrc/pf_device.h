@@ -37,23 +37,25 @@ struct FormMixed {
 
 template <class C> __host__ __device__ inline C conc_hi(C c) { return c; }
 
-// Mixed-form Kahan compensation, stored as a 16-bit fixed-point multiple of the value's ulp:
-// comp = q * 2^(e - 39) where c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. steps of 2^-15 ulp(c)
-// (ulp(c) = 2^(e - 24)).  |comp| <= ulp/2 after a Kahan update, so |q| <= 2^14 with 2x headroom;
-// codes saturate at the int16 range.  The effective concentration is c - comp.
-constexpr int kCompShift = 39;
+// Mixed-form Kahan compensation, stored in a 16-bit code with the cell's bulk flag:
+// code = q * 4 + flag, comp = q * 2^(e - 37) where c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. a
+// 14-bit signed multiple of 2^-13 ulp(c) (ulp(c) = 2^(e - 24)).  |comp| <= ulp/2 after a Kahan
+// update, so |q| <= 2^12 with 2x headroom; q saturates at [-8192, 8191].  The effective
+// concentration is c - comp.  The two flag bits (the bulk phase of the cell, 0 = full formula)
+// are set by the per-sweep classification and carried unchanged by every sub-step.
+constexpr int kCompShift = 37;
 __host__ __device__ inline int comp_exp(float c) {
   int e = 0;
   (void)frexpf(c, &e);
   return e;
 }
-__host__ __device__ inline float comp_decode(int16_t q, float c) {
-  return ldexpf(float(q), comp_exp(c) - kCompShift);
+__host__ __device__ inline float comp_decode(int16_t code, float c) {
+  return ldexpf(float(int(code) >> 2), comp_exp(c) - kCompShift);
 }
-__host__ __device__ inline int16_t comp_encode(float comp, float c) {
+__host__ __device__ inline int16_t comp_encode(float comp, float c, int flag_bits) {
   float v = rintf(ldexpf(comp, kCompShift - comp_exp(c)));
-  v = fminf(fmaxf(v, -32768.f), 32767.f);
-  return int16_t(v);
+  v = fminf(fmaxf(v, -8192.f), 8191.f);
+  return int16_t(int(v) * 4 + (flag_bits & 3));
 }
 // effective stored concentration of cell i of a buffer of n cells (mixed: value - compensation)
 template <class F> __device__ inline double conc_effective(const typename F::C *buf, long i, long n);

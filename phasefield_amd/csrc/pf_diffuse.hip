@@ -85,11 +85,12 @@ __device__ inline void conc_update<FormF64>(const double *__restrict__, double *
 template <>
 __device__ inline void conc_update<FormMixed>(const float *__restrict__ in, float *__restrict__ out,
                                               long i, long n, float c, float del) {
-  const float comp = comp_decode(comp_codes(in, n)[i], c);
+  const int16_t code = comp_codes(in, n)[i];
+  const float comp = comp_decode(code, c);
   const float y = del - comp;
   const float t = c + y;
   out[i] = t;
-  comp_codes(out, n)[i] = comp_encode((t - c) - y, t);
+  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, code);
 }
 
 // The reference's sub-step for one cell from its own and its stencil's stored concentration and
@@ -283,7 +284,9 @@ template <class T, int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfrac,
                                                        uint8_t *__restrict__ flag, const int nx,
                                                        const int ny, const int nz,
-                                                       const long ncell, const unsigned mask) {
+                                                       const long ncell, const unsigned mask,
+                                                       int16_t *__restrict__ codes0,
+                                                       int16_t *__restrict__ codes1) {
   const long cell = long(blockIdx.x) * 256 + threadIdx.x;
   if (cell >= ncell) return;
   const int x = int(cell % nx);
@@ -322,6 +325,9 @@ __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfr
       if (unit(nb[f]) != p) p = 0;
   }
   flag[cell] = uint8_t(p);
+  // mixed form: the flag also lives in the low bits of each solute's compensation code
+  if (codes0) codes0[cell] = int16_t((codes0[cell] & ~3) | p);
+  if (codes1) codes1[cell] = int16_t((codes1[cell] & ~3) | p);
 }
 
 // Bulk-aware sub-step: a lane owns one (x, y) column of a 64 x BY tile and walks a z-chunk, U
@@ -450,11 +456,11 @@ template <int V> struct alignas(2 * V) VecCode {
 template <class F, int V>
 __device__ inline void vec_update(const typename F::C *__restrict__ in, typename F::C *__restrict__ out,
                                   long i, long n, const VecA<typename F::A, V> &c,
-                                  const typename F::A (&del)[V]);
+                                  const typename F::A (&del)[V], const VecCode<V> &codes);
 template <>
 __device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double *__restrict__ out,
                                               long i, long, const VecA<double, 2> &c,
-                                              const double (&del)[2]) {
+                                              const double (&del)[2], const VecCode<2> &) {
   VecA<double, 2> o;
   o.v[0] = c.v[0] + del[0];
   o.v[1] = c.v[1] + del[1];
@@ -463,8 +469,7 @@ __device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double
 template <>
 __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, float *__restrict__ out,
                                                 long i, long n, const VecA<float, 4> &c,
-                                                const float (&del)[4]) {
-  const VecCode<4> qi = *reinterpret_cast<const VecCode<4> *>(comp_codes(in, n) + i);
+                                                const float (&del)[4], const VecCode<4> &qi) {
   VecA<float, 4> o;
   VecCode<4> qo;
 #pragma unroll
@@ -473,7 +478,7 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
     const float y = del[v] - comp;
     const float t = c.v[v] + y;
     o.v[v] = t;
-    qo.q[v] = comp_encode((t - c.v[v]) - y, t);
+    qo.q[v] = comp_encode((t - c.v[v]) - y, t, qi.q[v]);
   }
   *reinterpret_cast<VecA<float, 4> *>(out + i) = o;
   *reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i) = qo;
@@ -515,11 +520,19 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
     VA cc[U], ym[U], yp[U];
     A xm[U], xp[U];
     VecFlag<V> fl[U];
+    VecCode<V> qc[U];  // mixed: compensation codes, whose low bits carry the flags
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long base = plane * min(z + u, z1 - 1);
       const typename F::C *__restrict__ pl = cin + base;
-      fl[u] = *reinterpret_cast<const VecFlag<V> *>(flag + base + oc);
+      if constexpr (F::kMixed) {
+        qc[u] = *reinterpret_cast<const VecCode<V> *>(
+            reinterpret_cast<const int16_t *>(cin + ncell) + base + oc);
+#pragma unroll
+        for (int v = 0; v < V; ++v) fl[u].f[v] = uint8_t(qc[u].q[v] & 3);
+      } else {
+        fl[u] = *reinterpret_cast<const VecFlag<V> *>(flag + base + oc);
+      }
       cc[u] = ldv(pl, oc);
       xm[u] = A(pl[oxm]);
       xp[u] = A(pl[oxp]);
@@ -562,7 +575,7 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
           }
           del[v] = d * dc.r;
         }
-        vec_update<F, V>(cin, cout, plane * (z + u) + oc, ncell, cc[u], del);
+        vec_update<F, V>(cin, cout, plane * (z + u) + oc, ncell, cc[u], del, qc[u]);
       }
     }
   }
@@ -709,11 +722,11 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
 // explicit instantiations used by the host dispatch (pf_api.hip)
 #define PF_INST_CLASSIFY(T, P)                                                                \
   template __global__ void classify_kernel<T, P, 4, false>(const T *, uint8_t *, int, int, int,  \
-                                                           long, unsigned);                     \
+                                                           long, unsigned, int16_t *, int16_t *); \
   template __global__ void classify_kernel<T, P, 6, false>(const T *, uint8_t *, int, int, int,  \
-                                                           long, unsigned);                     \
+                                                           long, unsigned, int16_t *, int16_t *); \
   template __global__ void classify_kernel<T, P, 6, true>(const T *, uint8_t *, int, int, int,   \
-                                                          long, unsigned);
+                                                          long, unsigned, int16_t *, int16_t *);
 PF_INST_CLASSIFY(double, 2)
 PF_INST_CLASSIFY(double, 3)
 PF_INST_CLASSIFY(float, 2)

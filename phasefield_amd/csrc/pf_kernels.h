@@ -73,9 +73,12 @@ __global__ void general_scatter_kernel(const uint8_t *__restrict__ flag, long nc
                                        const int *__restrict__ boff, int *__restrict__ list);
 
 template <class T, int P, int NF, bool NOFLUX>
+// codes0/codes1: mixed form, the compensation codes of the solutes' current buffers (their low
+// two bits receive the flag); null for FP64
 __global__ void classify_kernel(const T *__restrict__ pfrac, uint8_t *__restrict__ flag,
                                 const int nx, const int ny, const int nz, const long ncell,
-                                const unsigned mask);
+                                const unsigned mask, int16_t *__restrict__ codes0,
+                                int16_t *__restrict__ codes1);
 
 template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
 __global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,
