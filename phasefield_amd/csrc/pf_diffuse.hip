@@ -26,103 +26,9 @@
 // A phase absent from a cell (phi_p == 0) gives zero weight on all its faces, so its partition is
 // not computed; weights sqrt(0) and sqrt(1) are exact and skip the square-root sequence; the weight
 // of a z face is shared by the two planes it separates (the product is commutative).
-#include "pf_kernels.h"
+#include "pf_diffuse_common.h"
 
 namespace pfamd {
-
-template <class A> struct CellPF {
-  A c[3];
-  A ph[3];
-};
-
-template <class F, int P> struct RawCell {
-  typename F::C c;
-  typename F::T ph[P];
-};
-
-template <class F, int P>
-__device__ inline void partition_cell(const RawCell<F, P> &r, const DiffConsts<typename F::A> &dc,
-                                      CellPF<typename F::A> &o) {
-  using A = typename F::A;
-  const A c = A(conc_hi(r.c));
-#pragma unroll
-  for (int j = 0; j < P; ++j) o.ph[j] = A(r.ph[j]);
-#pragma unroll
-  for (int i = 0; i < P; ++i) {
-    A v = A(0);
-    if (o.ph[i] > A(0)) {
-      A t1 = A(0), t2 = A(0);
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        t1 += o.ph[j] * dc.pk[i][j];
-        t2 += o.ph[j] * dc.pc[i][j];
-      }
-      v = c - t2;
-      if (t1 != A(1)) v = v / t1;
-    }
-    o.c[i] = v;
-  }
-}
-
-template <class A> __device__ inline A face_weight(A prod) {
-  A w = prod;
-  if (prod != A(0) && prod != A(1)) w = sqrt(prod);
-  return w;
-}
-
-// total_conc += del for cell i (solver.cpp:366) from its stored value c read from `in` (n cells
-// per buffer).  Mixed form: Kahan-compensated, y = del - comp; t = c + y; comp = (t - c) - y;
-// c = t, with the compensation code stored after the n values of the buffer.
-template <class F>
-__device__ inline void conc_update(const typename F::C *__restrict__ in,
-                                   typename F::C *__restrict__ out, long i, long n,
-                                   typename F::C c, typename F::A del);
-template <>
-__device__ inline void conc_update<FormF64>(const double *__restrict__, double *__restrict__ out,
-                                            long i, long, double c, double del) {
-  out[i] = c + del;
-}
-template <>
-__device__ inline void conc_update<FormMixed>(const float *__restrict__ in, float *__restrict__ out,
-                                              long i, long n, float c, float del) {
-  const int16_t code = comp_codes(in, n)[i];
-  const float comp = comp_decode(code, c);
-  const float y = del - comp;
-  const float t = c + y;
-  out[i] = t;
-  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, code);
-}
-
-// The reference's sub-step for one cell from its own and its stencil's stored concentration and
-// value_phase: partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365), update.
-template <class F, int P, int NF>
-__device__ inline void full_update(const typename F::C *__restrict__ cin,
-                                   typename F::C *__restrict__ cout,
-                                   const typename F::T *__restrict__ pfrac, long i, long n,
-                                   const long (&nbi)[6], const DiffConsts<typename F::A> &dc) {
-  using A = typename F::A;
-  RawCell<F, P> r;
-  r.c = cin[i];
-#pragma unroll
-  for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
-  CellPF<A> me;
-  partition_cell<F, P>(r, dc, me);
-  A del = A(0);
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    RawCell<F, P> rn;
-    rn.c = cin[nbi[f]];
-#pragma unroll
-    for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
-    CellPF<A> nb;
-    partition_cell<F, P>(rn, dc, nb);
-#pragma unroll
-    for (int p = 0; p < P; ++p)
-      del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
-  }
-  del *= dc.r;
-  conc_update<F>(cin, cout, i, n, r.c, del);
-}
 
 template <class F, int P, int NF, bool NOFLUX, int TY>
 __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
@@ -443,16 +349,6 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
 // x neighbours at the ends of its run with two scalar loads, and walks a z-chunk U planes at a
 // time.  No branch on the flag: cells with flag 0 get a value that diffuse_general_kernel
 // overwrites afterwards (same stream), so every store is a full vector.  Needs nx % V == 0.
-template <class A, int V> struct alignas(sizeof(A) * V) VecA {
-  A v[V];
-};
-template <int V> struct alignas(V) VecFlag {
-  uint8_t f[V];
-};
-template <int V> struct alignas(2 * V) VecCode {
-  int16_t q[V];
-};
-
 template <class F, int V>
 __device__ inline void vec_update(const typename F::C *__restrict__ in, typename F::C *__restrict__ out,
                                   long i, long n, const VecA<typename F::A, V> &c,
