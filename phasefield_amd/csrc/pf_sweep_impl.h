@@ -17,8 +17,9 @@
 // 0.5/dims*dx*dx (passed as dt) and no phase pair is skipped.
 // The mixed form accumulates the per-entry update, clamps, prunes, sorts and renormalises in FP64.
 //
-// The per-thread union lists (ids, own phi, neighbour sums, updated values) live in LDS as
-// [entry][thread] arrays so the data-dependent match/append/sort indices stay cheap.
+// The per-thread union lists (16-bit ids, neighbour sums, updated values) live in LDS as
+// [entry][thread] arrays so the data-dependent match/append/sort indices stay cheap; an entry's
+// own value is read from the thread's slot registers.
 #pragma once
 #include "pf_kernels.h"
 
@@ -46,8 +47,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     const long ncell, const int *__restrict__ list, const int *__restrict__ count) {
   using T = typename F::T;
   using A = typename F::A;
-  __shared__ uint32_t sLg[kStencilMax][kSweepBS];
-  __shared__ T sV1[kStencilMax][kSweepBS];
+  __shared__ uint16_t sLg[kStencilMax][kSweepBS];
   __shared__ T sV2[kStencilMax][kSweepBS];
   __shared__ double sDel[kStencilMax][kSweepBS];
   const int t = threadIdx.x;
@@ -61,7 +61,6 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
 
   for (int e = 0; e < kStencilMax; ++e) {
     sLg[e][t] = 0;
-    sV1[e][t] = T(0);
     sV2[e][t] = T(0);
   }
   uint32_t g0[K];
@@ -74,10 +73,19 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   for (int s = 0; s < K; ++s) {
     if (double(v0[s]) > cutoff) {
       ++n;
-      sLg[s][t] = g0[s];
-      sV1[s][t] = v0[s];
+      sLg[s][t] = uint16_t(g0[s]);
     }
   }
+
+  // the entry values of the own slots: an entry's own value is the slot value at its position
+  // when that slot is above the cutoff (positions >= K hold neighbour-only entries: 0)
+  auto v1_of = [&](int e) -> T {
+    T r = T(0);
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+      if (e == s && double(v0[s]) > cutoff) r = v0[s];
+    return r;
+  };
 
   long nb[6];
   {
@@ -117,7 +125,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
           if (kk < kStencilMax) {
             sV2[kk][t] += v;
           } else if (n < kStencilMax) {
-            sLg[n][t] = g;
+            sLg[n][t] = uint16_t(g);
             sV2[n][t] += v;
             ++n;
           }
@@ -171,11 +179,11 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   for (int a = 0; a < kStencilMax; ++a) sDel[a][t] = 0.0;
   for (int a = 0; a < n; ++a) {
     const int pa = code_phase(sLg[a][t]);
-    const A v1a = A(sV1[a][t]), v2a = A(sV2[a][t]);
+    const A v1a = A(v1_of(a)), v2a = A(sV2[a][t]);
     double acc = 0.0;
     for (int b = 0; b < n; ++b) {
       if (b == a) continue;
-      const A v1b = A(sV1[b][t]), v2b = A(sV2[b][t]);
+      const A v1b = A(v1_of(b)), v2b = A(sV2[b][t]);
       if (ISO) {
         const A tt = (v2a * v1b - v2b * v1a) * cs.f1 + (v1a - v1b) * cs.f2;
         acc += double(tt);
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     if (m != a) {
       sDel[m][t] = sDel[a][t];
       sDel[a][t] = best;
-      const uint32_t tg = sLg[a][t];
+      const uint16_t tg = sLg[a][t];
       sLg[a][t] = sLg[m][t];
       sLg[m][t] = tg;
     }
