@@ -81,11 +81,13 @@ def measured_traffic(workload: str) -> dict:
     return {}
 
 
-def diffusion_kernel_name(mixed: bool, nphase: int, ncell: int) -> str:
-    """The sub-step kernel the library launches for this domain (periodic 3D, default choice)."""
+def diffusion_kernel_name(mixed: bool, nphase: int, noflux: bool) -> str:
+    """The sub-step kernels the library launches for a 3D domain whose nx is a multiple of the
+    16-byte run (tile rows 8 or 16, picked by timing at the handle's first sub-step)."""
     form = "FormMixed" if mixed else "FormF64"
-    return (f"diffuse_bulk_vec_kernel<{form},{nphase},6,false,4,{4 if mixed else 2}> + "
-            f"diffuse_general_kernel<{form},{nphase},6,false> (one carbon sub-step)")
+    nof = "true" if noflux else "false"
+    return (f"diffuse_bulk_march_kernel<{form},{nphase},6,{nof},8|16,{4 if mixed else 2}> + "
+            f"diffuse_general_kernel<{form},{nphase},6,{nof}> (one carbon sub-step)")
 
 
 def timed_run(dom, model, steps: int, warmup: int):
@@ -353,7 +355,7 @@ def main() -> int:
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(achieved / HBM_PEAK_GBS, 4),
                                   "traffic": traffic.get("FormMixed" if name == "mixed" else "FormF64"),
-                                  "kernel": diffusion_kernel_name(name == "mixed", d.nphase, N),
+                                  "kernel": diffusion_kernel_name(name == "mixed", d.nphase, d.bc_z == 1),
                                   "bytes_per_cell": bpc, "cells_per_launch": N,
                                   "launch_us": round(launch_us, 2)}}
         log(f"{name}: {res[name]['ms_per_step']:.2f} ms/step, diffusion {launch_us:.1f} us "

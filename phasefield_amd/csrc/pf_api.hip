@@ -65,6 +65,9 @@ struct pf_handle {
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
   bool diff_split = true;    // full-formula cells from a per-sweep list in their own kernel
   bool diff_vec = true;      // bulk-only sub-step with 16-byte runs per lane (when nx allows)
+  bool diff_march = true;    // ... in its plane-marching form
+  int march_by = 0;          // its tile rows and z-chunk, picked by timing at first use
+  int march_zc = 0;
   int *glist = nullptr;      // indices of the cells with flag 0, storage order
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
   bool sweep_split = true;   // copy cells classified and written apart from the listed sweep
@@ -266,6 +269,84 @@ DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk, int
   return g;
 }
 
+// The plane-marching bulk kernel.  Its speed depends on how many workgroups are resident per CU
+// and how long each one marches (at 512^3 the mixed form's sub-step ranges over 357-432 us across
+// tile heights and z-chunks, and the best shape differs between the forms and sizes), which no
+// closed rule captured.  So at the first sub-step of a handle each candidate tile height (8, 16
+// rows) and z-chunk (8..128 planes) is launched and timed into the output buffer, and the fastest
+// is kept.  Every candidate writes the same bits
+// (bulk cells; the flag-0 cells are rewritten by the listed pass), so the choice changes speed
+// only.  PFAMD_DIFF_BY (8 or 16) and PFAMD_DIFF_ZCHUNK pin either choice.
+template <class F, int P, int NF, bool NOF>
+hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
+                        const typename F::C *cin, typename F::C *cout) {
+  constexpr int V = F::kMixed ? 4 : 2;
+  const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz;
+  auto launch = [&](int by, int zc) -> hipError_t {
+    const int tiles_x = (nx + 64 * V - 1) / (64 * V), tiles_y = (ny + by - 1) / by;
+    const long nblocks = long(tiles_x) * tiles_y * ((nz + zc - 1) / zc);
+    if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    if (by == 8)
+      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 8, V>), dim3(nblocks),
+                         dim3(64, 8), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
+                         tiles_x, tiles_y, int(nblocks));
+    else
+      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 16, V>), dim3(nblocks),
+                         dim3(64, 16), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
+                         tiles_x, tiles_y, int(nblocks));
+    return hipGetLastError();
+  };
+  if (h->march_by == 0) {
+    std::vector<int> bys, zcs;
+    if (h->diff_by == 8 || h->diff_by == 16) bys = {h->diff_by};
+    else bys = {8, 16};
+    if (h->diff_zchunk_override > 0) {
+      zcs = {std::min(h->diff_zchunk_override, nz)};
+    } else {
+      for (int zc : {8, 16, 32, 64, 128}) {
+        const int c = std::min(zc, nz);
+        if (std::find(zcs.begin(), zcs.end(), c) == zcs.end()) zcs.push_back(c);
+      }
+    }
+    if (bys.size() * zcs.size() == 1) {
+      h->march_by = bys[0];
+      h->march_zc = zcs[0];
+    } else {
+      hipEvent_t e0, e1;
+      hipError_t err = hipEventCreate(&e0);
+      if (err != hipSuccess) return err;
+      if ((err = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return err;
+      }
+      float best = 0.f;
+      for (int by : bys)
+        for (int zc : zcs) {
+          float ms = 0.f;
+          if ((err = launch(by, zc)) != hipSuccess) break;  // warm-up
+          if ((err = hipEventRecord(e0, h->stream)) != hipSuccess) break;
+          for (int r = 0; r < 2 && err == hipSuccess; ++r) err = launch(by, zc);
+          if (err != hipSuccess) break;
+          if ((err = hipEventRecord(e1, h->stream)) != hipSuccess) break;
+          if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+          if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+          if (h->march_by == 0 || ms < best) {
+            best = ms;
+            h->march_by = by;
+            h->march_zc = zc;
+          }
+        }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (err != hipSuccess) {
+        h->march_by = h->march_zc = 0;
+        return err;
+      }
+    }
+  }
+  return launch(h->march_by, h->march_zc);
+}
+
 // one sub-step from `cin` into `cout` with the handle's kernel choice and bulk flags
 template <class F, int P, int NF, bool NOF>
 hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
@@ -323,7 +404,12 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
         go(std::integral_constant<int, 4>(), split);
     };
     constexpr int V = F::kMixed ? 4 : 2;  // 16-byte runs per lane
-    if (h->diff_split && h->diff_vec && h->d.nx % V == 0 && (h->diff_by < 0 || h->diff_by == 4)) {
+    if (h->diff_split && h->diff_vec && h->diff_march && h->d.nx % V == 0 &&
+        (h->diff_by < 0 || h->diff_by == 8 || h->diff_by == 16)) {
+      const hipError_t e = launch_march<F, P, NF, NOF>(h, dc, cin, cout);
+      if (e != hipSuccess) return e;
+    } else if (h->diff_split && h->diff_vec && h->d.nx % V == 0 &&
+               (h->diff_by < 0 || h->diff_by == 4)) {
       const DiffGeom g = diff_geom(h, 4, 131072 / 4, 8, 64 * V);
       hipLaunchKernelGGL((diffuse_bulk_vec_kernel<F, P, NF, NOF, 4, V>), dim3(g.nblocks),
                          dim3(64, 4), 0, h->stream, cin, cout, h->flag, dc, h->d.nx, h->d.ny,
@@ -471,6 +557,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if (const char *e = std::getenv("PFAMD_DIFF_BY")) h->diff_by = std::atoi(e);
   if (const char *e = std::getenv("PFAMD_DIFF_SPLIT")) h->diff_split = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_DIFF_VEC")) h->diff_vec = std::atoi(e) != 0;
+  if (const char *e = std::getenv("PFAMD_DIFF_MARCH")) h->diff_march = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_SWEEP_SPLIT")) h->sweep_split = std::atoi(e) != 0;
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {

@@ -57,6 +57,15 @@ __global__ void diffuse_bulk_vec_kernel(const typename F::C *__restrict__ cin,
                                         const int ny, const int nz, const int zchunk,
                                         const int tiles_x, const int tiles_y, const int nblocks);
 
+// plane-marching bulk kernel: same contract as diffuse_bulk_vec_kernel
+template <class F, int P, int NF, bool NOFLUX, int BY, int V>
+__global__ void diffuse_bulk_march_kernel(const typename F::C *__restrict__ cin,
+                                          typename F::C *__restrict__ cout,
+                                          const uint8_t *__restrict__ flag,
+                                          const DiffConsts<typename F::A> dc, const int nx,
+                                          const int ny, const int nz, const int zchunk,
+                                          const int tiles_x, const int tiles_y, const int nblocks);
+
 template <class F, int P, int NF, bool NOFLUX>
 __global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
                                        typename F::C *__restrict__ cout,

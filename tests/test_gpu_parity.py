@@ -121,28 +121,51 @@ def test_full_steps_bitwise(tmp_path, monkeypatch, shape, bc, K, sweep_split):
 
 
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
-@pytest.mark.parametrize("kernel,rows,split,vec", [
-    ("bulk", "-1", "1", "1"),
-    ("bulk", "0", "1", "0"), ("bulk", "4", "1", "0"), ("bulk", "8", "1", "0"),
-    ("bulk", "16", "1", "0"),
-    ("bulk", "0", "0", "0"), ("bulk", "4", "0", "0"), ("bulk", "8", "0", "0"),
-    ("bulk", "16", "0", "0"),
-    ("lds", "-1", "0", "0")])
-def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, split, vec, form):
-    """Every diffusion kernel (16-byte-run bulk kernel; storage order and column-marching with
-    4/8/16-row tiles, each with the full-formula cells inline or in their own listed pass;
-    LDS-tiled full formula) against the oracle on a seeded microstructure (bulk and interface
-    cells), in both forms."""
-    banner(f"diffusion variant {kernel} rows={rows} split={split} vec={vec} form={form}")
+@pytest.mark.parametrize("kernel,rows,split,vec,march,zchunk", [
+    ("bulk", "-1", "1", "1", "1", "0"),
+    ("bulk", "8", "1", "1", "1", "5"), ("bulk", "16", "1", "1", "1", "3"),
+    ("bulk", "-1", "1", "1", "0", "0"),
+    ("bulk", "0", "1", "0", "0", "0"), ("bulk", "4", "1", "0", "0", "0"),
+    ("bulk", "8", "1", "0", "0", "0"), ("bulk", "16", "1", "0", "0", "0"),
+    ("bulk", "0", "0", "0", "0", "0"), ("bulk", "4", "0", "0", "0", "0"),
+    ("bulk", "8", "0", "0", "0", "0"), ("bulk", "16", "0", "0", "0", "0"),
+    ("lds", "-1", "0", "0", "0", "0")])
+def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, split, vec, march,
+                                           zchunk, form):
+    """Every diffusion kernel against the oracle on a seeded microstructure (bulk and interface
+    cells), in both forms: the plane-marching bulk kernel (timed choice of tile and z-chunk, and
+    8/16-row tiles with 5/3-plane chunks), the batched 16-byte-run bulk kernel, storage order and
+    column-marching with 4/8/16-row tiles (full-formula cells inline or in their own listed
+    pass), and the LDS-tiled full formula."""
+    banner(f"diffusion variant {kernel} rows={rows} split={split} vec={vec} march={march} "
+           f"zchunk={zchunk} form={form}")
     monkeypatch.setenv("PFAMD_DIFF_KERNEL", kernel)
     monkeypatch.setenv("PFAMD_DIFF_BY", rows)
     monkeypatch.setenv("PFAMD_DIFF_SPLIT", split)
     monkeypatch.setenv("PFAMD_DIFF_VEC", vec)
+    monkeypatch.setenv("PFAMD_DIFF_MARCH", march)
+    monkeypatch.setenv("PFAMD_DIFF_ZCHUNK", zchunk)
     shape = (72, 36, 22)  # nx a multiple of 4 and not of 64: partial vector tiles
     m, p, st = _setup_state(tmp_path, nx=shape[0], ny=shape[1], nz=shape[2], grains=6, nuclei=5)
     gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p,
                        lambda g, q: g.step(2), lambda o, q: o.step(2))
-    assert_same(gs, os_, 4, f"{kernel}/{rows}/{split}/{vec}")
+    assert_same(gs, os_, 4, f"{kernel}/{rows}/{split}/{vec}/{march}/{zchunk}")
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("rows,zchunk", [("8", "5"), ("16", "4"), ("-1", "0")])
+def test_march_noflux_bitwise(tmp_path, monkeypatch, rows, zchunk, form):
+    """The plane-marching bulk kernel with no-flux z faces: chunk end planes clamp to the cell
+    itself at the domain's z ends and read the neighbour chunk's plane inside."""
+    banner(f"march noflux rows={rows} zchunk={zchunk} form={form}")
+    monkeypatch.setenv("PFAMD_DIFF_BY", rows)
+    monkeypatch.setenv("PFAMD_DIFF_ZCHUNK", zchunk)
+    shape = (40, 36, 24)
+    m, p, st = _setup_state(tmp_path, nx=shape[0], ny=shape[1], nz=shape[2], grains=6, nuclei=4,
+                            bc_z="noflux")
+    gs, os_ = run_pair(shape, 4, 2, 1, 1, form, st, p,
+                       lambda g, q: g.step(2), lambda o, q: o.step(2))
+    assert_same(gs, os_, 4, f"march noflux {rows}/{zchunk}")
 
 
 def test_bulk_reduction_needs_identity_partition(tmp_path):
