@@ -722,9 +722,22 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, N, ablock,
                            alist);
       }
+      // the forward step's plane-marching shape, when its first sub-step has picked one
+      int mby = 0, mzc = 0;
+      handle_march_shape(h, &mby, &mzc);
+      const int mtx = (v.nx + 127) / 128, mty = mby ? (v.ny + mby - 1) / mby : 0;
+      const long mnb = mby ? long(mtx) * mty * ((v.nz + mzc - 1) / mzc) : 0;
       for (int s = nsub - 1; s >= 0; --s) {
         if (split) {
-          if (vec)
+          if (vec && mby == 8)
+            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 8, 2>), dim3(mnb),
+                               dim3(64, 8), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx, v.ny, v.nz,
+                               mzc, mtx, mty, int(mnb));
+          else if (vec && mby == 16)
+            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 16, 2>), dim3(mnb),
+                               dim3(64, 16), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx, v.ny, v.nz,
+                               mzc, mtx, mty, int(mnb));
+          else if (vec)
             hipLaunchKernelGGL((diffuse_bulk_vec_kernel<FormF64, P, NF, NOF, 4, 2>),
                                dim3(gvec.nblocks), dim3(64, 4), 0, st, lam_c, lam_c2, sd.flag, dc,
                                v.nx, v.ny, v.nz, gvec.zchunk, gvec.tiles_x, gvec.tiles_y,

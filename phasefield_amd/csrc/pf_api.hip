@@ -1316,6 +1316,11 @@ HandleView handle_view(pf_handle *h) {
   return v;
 }
 int handle_fail(pf_handle *h, int code, const std::string &msg) { return fail(h, code, msg); }
+void handle_march_shape(const pf_handle *h, int *rows, int *zchunk) {
+  const bool used = h->diff_split && h->diff_vec && h->diff_march && h->diff_variant == 0;
+  *rows = used ? h->march_by : 0;
+  *zchunk = used ? h->march_zc : 0;
+}
 
 namespace {
 template <int P>

@@ -28,6 +28,9 @@ int handle_fail(pf_handle *h, int code, const std::string &msg);
 void handle_state_changed(pf_handle *h);
 pfamd::PFConsts<double> handle_pf_consts(const pf_handle *h);
 pfamd::DiffConsts<double> handle_diff_consts(const pf_handle *h, int solute, double factor);
+// tile rows and z-chunk the plane-marching bulk kernel runs with on this handle (0, 0 until its
+// first sub-step has picked them, or when that kernel is not in use)
+void handle_march_shape(const pf_handle *h, int *rows, int *zchunk);
 // one FP64 carbon sub-step from cin into cout with the step's own kernel, flags and factor
 // (value_phase and bulk flags of the handle's current state)
 int handle_diffuse_io(pf_handle *h, const double *cin, double *cout);
