@@ -594,6 +594,7 @@ template <int K, int P, int NF, bool NOF> struct Adj {
     const bool vec = v.nx % 2 == 0;
     const unsigned nl = unsigned((N + 256L * kListChunk - 1) / (256L * kListChunk));
     const unsigned ngen = std::min<unsigned>(gb, 2048);
+    uint8_t *unit_tmp = ar.get<uint8_t>(N);  // scratch of the bulk classification
     int *alist = split ? ar.get<int>(N) : nullptr;
     int *ablock = split ? ar.get<int>(nl + 1) : nullptr;
     const TileGeom gvec = tile_geom(v.nx, v.ny, v.nz, NF, 128);
@@ -695,9 +696,8 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       if (r != PF_OK) return r;
       const DevState cur = on_handle();
       hipMemcpyAsync(sd.pf_new, cur.pf, n8 * P, hipMemcpyDeviceToDevice, st);
-      hipLaunchKernelGGL((classify_kernel<double, P, NF, NOF>), dim3(gb), dim3(256), 0, st,
-                         sd.pf_new, sd.flag, v.nx, v.ny, v.nz, N, v.bulk_mask, nullptr,
-                         nullptr);
+      (void)launch_classify<double, P, NF, NOF>(st, sd.pf_new, unit_tmp, sd.flag, v.nx, v.ny,
+                                                 v.nz, N, v.bulk_mask, nullptr, nullptr);
       hipMemcpyAsync(sd.csub, cur.c, n8, hipMemcpyDeviceToDevice, st);
       for (int s = 0; s < nsub && r == PF_OK; ++s)
         r = handle_diffuse_io(h, sd.csub + size_t(N) * s, sd.csub + size_t(N) * (s + 1));

@@ -60,6 +60,7 @@ struct pf_handle {
   int diff_zchunk_override = 0;
   int diff_variant = 0;  // 0: bulk-aware column kernel, 1: LDS-tiled full-formula kernel
   uint8_t *flag = nullptr;   // per-cell bulk phase (0 = needs the full formula)
+  uint8_t *unit = nullptr;   // scratch of the classification: per-cell unit-vector phase
   bool flag_valid = false;   // flags describe the current value_phase and stencil
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
@@ -365,9 +366,10 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
       if (F::kMixed)
         for (int q = 0; q < h->d.nsolute; ++q)
           codes[q] = comp_codes(static_cast<float *>(h->conc[q][h->ccur[q]]), h->N);
-      hipLaunchKernelGGL((classify_kernel<T, P, NF, NOF>), dim3(nb), dim3(256), 0, h->stream, pf,
-                         h->flag, h->d.nx, h->d.ny, h->d.nz, h->N, h->bulk_mask, codes[0],
-                         codes[1]);
+      const hipError_t ec =
+          launch_classify<T, P, NF, NOF>(h->stream, pf, h->unit, h->flag, h->d.nx, h->d.ny,
+                                         h->d.nz, h->N, h->bulk_mask, codes[0], codes[1]);
+      if (ec != hipSuccess) return ec;
       if (h->diff_split) {
         const unsigned nl = list_blocks(h->N);
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, h->N,
@@ -492,6 +494,7 @@ void free_all(pf_handle *h) {
   }
   if (h->pfrac) hipFree(h->pfrac);
   if (h->flag) hipFree(h->flag);
+  if (h->unit) hipFree(h->unit);
   if (h->glist) hipFree(h->glist);
   if (h->sflag) hipFree(h->sflag);
   if (h->gblock) hipFree(h->gblock);
@@ -579,6 +582,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if ((rc = chk(hipMalloc(&h->pfrac, N * d.nphase * h->tsz), "alloc value_phase")) != PF_OK)
     return rc;
   if ((rc = chk(hipMalloc(&h->flag, N), "alloc bulk flags")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->unit, N), "alloc bulk flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->sflag, N), "alloc sweep copy flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->gblock, (list_blocks(h->N) + 1) * sizeof(int)),

@@ -8,8 +8,9 @@
 // -ffp-contract=off, and HIP's division and square root are IEEE correctly rounded on gfx950.
 //
 // Kernels in this file:
-//   classify_kernel        once per phase-field sweep: per-cell flag of "bulk" cells whose whole
-//                          stencil sits in one phase (exact reduction of the formula, see below)
+//   unit_phase_kernel, classify_kernel  once per phase-field sweep: per-cell flag of "bulk" cells
+//                          whose whole stencil sits in one phase (exact reduction of the formula,
+//                          see below)
 //   diffuse_bulk_kernel    the default sub-step for large domains: column-marching, U planes in
 //                          flight per lane, bulk cells skip the partition and weights
 //   diffuse_linear_kernel  the same arithmetic, one thread per cell in storage order (small domains)
@@ -191,11 +192,37 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
 //   0 * finite = +-0 and leave the running sum unchanged (it starts at +0 and can never become
 //   -0 by round-to-nearest addition); the partition of phase p is (c - t2)/t1 with
 //   t1 = 1*k(p,p) + sum 0*k(p,j) = 1 and t2 = 1*c(p,p) + sum 0*c(p,j) = +-0, i.e. exactly c.
-template <class T, int P, int NF, bool NOFLUX>
-__global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfrac,
+//
+// Two passes: unit_phase_kernel streams value_phase once and writes one byte per cell (p when the
+// cell's value_phase is exactly e_p and bit p of `mask` is set, else 0); classify_kernel then
+// compares each cell's byte with its stencil's.  The bytes (1/16 of FP64 value_phase) are what the
+// stencil re-reads, from cache.
+template <class T, int P>
+__global__ __launch_bounds__(256) void unit_phase_kernel(const T *__restrict__ pfrac,
+                                                         const long ncell, const unsigned mask,
+                                                         uint8_t *__restrict__ unit) {
+  const long cell = long(blockIdx.x) * 256 + threadIdx.x;
+  if (cell >= ncell) return;
+  int p = 0, ones = 0, zeros = 0;
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const T v = pfrac[cell * P + q];
+    if (v == T(1)) {
+      ++ones;
+      p = q + 1;
+    } else if (v == T(0)) {
+      ++zeros;
+    }
+  }
+  if (!(ones == 1 && zeros == P - 1) || !((mask >> p) & 1u)) p = 0;
+  unit[cell] = uint8_t(p);
+}
+
+template <int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict__ unit,
                                                        uint8_t *__restrict__ flag, const int nx,
                                                        const int ny, const int nz,
-                                                       const long ncell, const unsigned mask,
+                                                       const long ncell,
                                                        int16_t *__restrict__ codes0,
                                                        int16_t *__restrict__ codes1) {
   const long cell = long(blockIdx.x) * 256 + threadIdx.x;
@@ -204,22 +231,7 @@ __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfr
   const long yz = cell / nx;
   const int y = int(yz % ny);
   const int z = int(yz / ny);
-  auto unit = [&](long c) -> int {  // p if value_phase(c) == e_p exactly, else 0
-    int p = 0, ones = 0, zeros = 0;
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-      const T v = pfrac[c * P + q];
-      if (v == T(1)) {
-        ++ones;
-        p = q + 1;
-      } else if (v == T(0)) {
-        ++zeros;
-      }
-    }
-    return (ones == 1 && zeros == P - 1) ? p : 0;
-  };
-  int p = unit(cell);
-  if (!((mask >> p) & 1u)) p = 0;
+  int p = unit[cell];
   if (p != 0) {
     const long plane = long(nx) * ny;
     long nb[6];
@@ -233,12 +245,24 @@ __global__ __launch_bounds__(256) void classify_kernel(const T *__restrict__ pfr
     nb[5] = x + long(nx) * y + plane * zp;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
-      if (unit(nb[f]) != p) p = 0;
+      if (unit[nb[f]] != p) p = 0;
   }
   flag[cell] = uint8_t(p);
   // mixed form: the flag also lives in the low bits of each solute's compensation code
   if (codes0) codes0[cell] = int16_t((codes0[cell] & ~3) | p);
   if (codes1) codes1[cell] = int16_t((codes1[cell] & ~3) | p);
+}
+
+template <class T, int P, int NF, bool NOFLUX>
+hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
+                           int ny, int nz, long ncell, unsigned mask, int16_t *codes0,
+                           int16_t *codes1) {
+  const unsigned nb = unsigned((ncell + 255) / 256);
+  hipLaunchKernelGGL((unit_phase_kernel<T, P>), dim3(nb), dim3(256), 0, st, pfrac, ncell, mask,
+                     unit);
+  hipLaunchKernelGGL((classify_kernel<NF, NOFLUX>), dim3(nb), dim3(256), 0, st, unit, flag, nx, ny,
+                     nz, ncell, codes0, codes1);
+  return hipGetLastError();
 }
 
 // Bulk-aware sub-step: a lane owns one (x, y) column of a 64 x BY tile and walks a z-chunk, U
@@ -794,13 +818,16 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
 }
 
 // explicit instantiations used by the host dispatch (pf_api.hip)
-#define PF_INST_CLASSIFY(T, P)                                                                \
-  template __global__ void classify_kernel<T, P, 4, false>(const T *, uint8_t *, int, int, int,  \
-                                                           long, unsigned, int16_t *, int16_t *); \
-  template __global__ void classify_kernel<T, P, 6, false>(const T *, uint8_t *, int, int, int,  \
-                                                           long, unsigned, int16_t *, int16_t *); \
-  template __global__ void classify_kernel<T, P, 6, true>(const T *, uint8_t *, int, int, int,   \
-                                                          long, unsigned, int16_t *, int16_t *);
+#define PF_INST_CLASSIFY(T, P)                                                                 \
+  template hipError_t launch_classify<T, P, 4, false>(hipStream_t, const T *, uint8_t *, uint8_t *, \
+                                                      int, int, int, long, unsigned, int16_t *,    \
+                                                      int16_t *);                                  \
+  template hipError_t launch_classify<T, P, 6, false>(hipStream_t, const T *, uint8_t *, uint8_t *, \
+                                                      int, int, int, long, unsigned, int16_t *,    \
+                                                      int16_t *);                                  \
+  template hipError_t launch_classify<T, P, 6, true>(hipStream_t, const T *, uint8_t *, uint8_t *,  \
+                                                     int, int, int, long, unsigned, int16_t *,     \
+                                                     int16_t *);
 PF_INST_CLASSIFY(double, 2)
 PF_INST_CLASSIFY(double, 3)
 PF_INST_CLASSIFY(float, 2)

@@ -81,13 +81,13 @@ __global__ void general_scan_kernel(int *__restrict__ bcount, int nb);
 __global__ void general_scatter_kernel(const uint8_t *__restrict__ flag, long ncell,
                                        const int *__restrict__ boff, int *__restrict__ list);
 
-template <class T, int P, int NF, bool NOFLUX>
+// Bulk flags of a value_phase field into `flag` (two kernels; `unit` is an N-byte scratch array).
 // codes0/codes1: mixed form, the compensation codes of the solutes' current buffers (their low
 // two bits receive the flag); null for FP64
-__global__ void classify_kernel(const T *__restrict__ pfrac, uint8_t *__restrict__ flag,
-                                const int nx, const int ny, const int nz, const long ncell,
-                                const unsigned mask, int16_t *__restrict__ codes0,
-                                int16_t *__restrict__ codes1);
+template <class T, int P, int NF, bool NOFLUX>
+hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
+                           int ny, int nz, long ncell, unsigned mask, int16_t *codes0,
+                           int16_t *codes1);
 
 template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
 __global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,
