@@ -38,10 +38,8 @@ namespace pfamd {
 constexpr int kAdjBS = 64;
 
 __device__ inline void nbrs(long cell, int nx, int ny, int nz, bool noflux, long (&nb)[6]) {
-  const int x = int(cell % nx);
-  const long yz = cell / nx;
-  const int y = int(yz % ny);
-  const int z = int(yz / ny);
+  int x, y, z;
+  cell_coords(cell, nx, ny, x, y, z);
   const long plane = long(nx) * ny;
   nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
   nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;

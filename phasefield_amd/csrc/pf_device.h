@@ -73,6 +73,17 @@ template <> __device__ inline double conc_effective<FormMixed>(const float *buf,
   return double(buf[i]) - double(comp_decode(comp_codes(buf, n)[i], buf[i]));
 }
 
+// (x, y, z) of a storage-order cell index.  Cell indices stay below 2^31 (pf_create rejects larger
+// domains), so 32-bit unsigned division replaces the much longer 64-bit division sequence.
+__device__ inline void cell_coords(long cell, int nx, int ny, int &x, int &y, int &z) {
+  const unsigned c = unsigned(cell), ux = unsigned(nx), uy = unsigned(ny);
+  const unsigned yz = c / ux;
+  const unsigned zz = yz / uy;
+  x = int(c - yz * ux);
+  y = int(yz - zz * uy);
+  z = int(zz);
+}
+
 // periodic wrap of i+d for |d| <= n (Get_neighbor, phase_field.h:58-68, 0-based)
 __device__ inline int wrap(int i, int n) {
   i = i < 0 ? i + n : i;

@@ -227,10 +227,8 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict
                                                        int16_t *__restrict__ codes1) {
   const long cell = long(blockIdx.x) * 256 + threadIdx.x;
   if (cell >= ncell) return;
-  const int x = int(cell % nx);
-  const long yz = cell / nx;
-  const int y = int(yz % ny);
-  const int z = int(yz / ny);
+  int x, y, z;
+  cell_coords(cell, nx, ny, x, y, z);
   int p = unit[cell];
   if (p != 0) {
     const long plane = long(nx) * ny;
@@ -690,10 +688,8 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
   const long plane = long(nx) * ny;
   const long i = long(blockIdx.x) * 256 + threadIdx.x;
   if (i >= plane * nz) return;
-  const int x = int(i % nx);
-  const long yz = i / nx;
-  const int y = int(yz % ny);
-  const int z = int(yz / ny);
+  int x, y, z;
+  cell_coords(i, nx, ny, x, y, z);
   long nbi[6];
   nbi[0] = i - x + wrap(x - 1, nx);
   nbi[1] = i - x + wrap(x + 1, nx);
@@ -736,10 +732,8 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
   const long plane = long(nx) * ny;
   for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
     const long i = list[k];
-    const int x = int(i % nx);
-    const long yz = i / nx;
-    const int y = int(yz % ny);
-    const int z = int(yz / ny);
+    int x, y, z;
+    cell_coords(i, nx, ny, x, y, z);
     long nbi[6];
     nbi[0] = i - x + wrap(x - 1, nx);
     nbi[1] = i - x + wrap(x + 1, nx);

@@ -53,10 +53,8 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   const int t = threadIdx.x;
   // no block-wide barrier below: threads run their cells independently
   auto body = [&](const long cell) {
-  const int x = int(cell % nx);
-  const long yz = cell / nx;
-  const int y = int(yz % ny);
-  const int z = int(yz / ny);
+  int x, y, z;
+  cell_coords(cell, nx, ny, x, y, z);
   const double cutoff = cs.cutoff;
 
   for (int e = 0; e < kStencilMax; ++e) {
@@ -295,10 +293,8 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
   bool copy = n0 <= 1;
   bool any = n0 == 1;
   if (copy) {
-    const int x = int(cell % nx);
-    const long yz = cell / nx;
-    const int y = int(yz % ny);
-    const int z = int(yz / ny);
+    int x, y, z;
+    cell_coords(cell, nx, ny, x, y, z);
     const long plane = long(nx) * ny;
     long nb[6];
     nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
