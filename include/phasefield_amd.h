@@ -234,6 +234,18 @@ int pf_rex_fraction(pf_handle *h, double *out);
 /* ASCII VTK ImageData dump (write_data.cpp:42-82) of a reference-layout host state. */
 int pf_write_vti(const char *path, int nx, int ny, int nz, int depth, double dx,
                  const int32_t *index, const double *conc);
+/* The same fields (korn as Int32, the concentration as Float32) as a binary VTK ImageData file
+ * with raw appended data.  No reference counterpart: the reference writes only the ASCII form. */
+int pf_write_vti_raw(const char *path, int nx, int ny, int nz, double dx, const int32_t *korn,
+                     const float *conc);
+/* Asynchronous field dump of the device state (the fields of pf_write_vti_raw, carbon): the
+ * fields are extracted on the device in stream order, copied to pinned host memory on a second
+ * stream and written by a host thread while the caller keeps stepping.  A second call, and
+ * pf_destroy, first wait for the previous dump.  No reference counterpart (write_data.cpp:42-82
+ * downloads and formats synchronously); it is the dump path for domains where that stalls. */
+int pf_dump_vti_async(pf_handle *h, const char *path, double dx);
+/* Wait for the last pf_dump_vti_async; returns its status (PF_OK when none is pending). */
+int pf_dump_wait(pf_handle *h);
 /* Full-precision binary dump / reload of a reference-layout host state. */
 int pf_write_state(const char *path, const pf_desc *d, double time, int64_t step,
                    const pf_host_state *s);

@@ -27,7 +27,8 @@ from ._capi import (PF_BC_NOFLUX, PF_BC_PERIODIC, PF_FP64, PF_MIXED, PfDesc, PfH
 
 __all__ = ["Domain", "Model", "PhaseField", "PfDesc", "PfParams", "PhaseFieldError",
            "PF_FP64", "PF_MIXED", "PF_BC_PERIODIC", "PF_BC_NOFLUX", "lib", "gpu_init",
-           "gpu_diffusion_solver", "gpu_cleanup", "params_from_dict", "params_to_dict"]
+           "gpu_diffusion_solver", "gpu_cleanup", "params_from_dict", "params_to_dict",
+           "write_vti_raw", "read_vti_raw"]
 
 
 def lib():
@@ -170,6 +171,15 @@ class Domain:
 
     def synchronize(self) -> None:
         _check(lib().pf_synchronize(self._h), self._h, "pf_synchronize")
+
+    def dump_vti_async(self, path: str, dx: float) -> None:
+        """Binary VTI dump (korn, carbon) of the current state, written by a host thread while
+        the caller keeps stepping; `dump_wait` returns when the file is complete."""
+        _check(lib().pf_dump_vti_async(self._h, path.encode(), float(dx)), self._h,
+               "pf_dump_vti_async")
+
+    def dump_wait(self) -> None:
+        _check(lib().pf_dump_wait(self._h), self._h, "pf_dump_wait")
 
     def reduce(self, check_finite: bool = True) -> dict:
         st = PfStats()
@@ -417,6 +427,38 @@ class PhaseField:
             return False
         st = self.domain.reduce()
         return self.model.write_kinetics(st, self.domain.rex_fraction(), True)
+
+
+# ---- binary field files --------------------------------------------------------------------
+def write_vti_raw(path: str, shape, dx: float, korn: np.ndarray, conc: np.ndarray) -> None:
+    """korn (int32) and conc (float32), storage order, as a binary VTI file (raw appended)."""
+    nx, ny, nz = shape
+    k = np.ascontiguousarray(korn, np.int32)
+    c = np.ascontiguousarray(conc, np.float32)
+    if k.size != nx * ny * nz or c.size != k.size:
+        raise ValueError("korn/conc size does not match the shape")
+    _check(lib().pf_write_vti_raw(path.encode(), nx, ny, nz, float(dx),
+                                  k.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  c.ctypes.data_as(C.POINTER(C.c_float))), None, "pf_write_vti_raw")
+
+
+def read_vti_raw(path: str) -> dict:
+    """Parse a binary VTI file of write_vti_raw / Domain.dump_vti_async: extent, spacing and the
+    korn / conc arrays (storage order)."""
+    import re
+    raw = open(path, "rb").read()
+    head, _, rest = raw.partition(b"<AppendedData encoding=\"raw\">\n_")
+    text = head.decode()
+    ext = [int(v) for v in re.search(r'WholeExtent="([^"]+)"', text).group(1).split()]
+    spacing = float(re.search(r'Spacing="([^ "]+)', text).group(1))
+    n0 = int(np.frombuffer(rest[:8], np.uint64)[0])
+    korn = np.frombuffer(rest[8:8 + n0], np.int32)
+    n1 = int(np.frombuffer(rest[8 + n0:16 + n0], np.uint64)[0])
+    conc = np.frombuffer(rest[16 + n0:16 + n0 + n1], np.float32)
+    tail = rest[16 + n0 + n1:].decode()
+    if "</AppendedData>" not in tail or "</VTKFile>" not in tail:
+        raise ValueError("truncated VTI file")
+    return {"shape": (ext[1], ext[3], ext[5]), "dx": spacing, "korn": korn, "conc": conc}
 
 
 # ---- the reference's accelerator API (gpu_solver.h:18-23) -----------------------------------

@@ -85,6 +85,8 @@ SIGNATURES = {
     "pf_diffuse_substeps": (_I, [_H, _I, _I, _D]),
     "pf_reduce": (_I, [_H, C.POINTER(PfStats)]),
     "pf_synchronize": (_I, [_H]),
+    "pf_dump_vti_async": (_I, [_H, C.c_char_p, _D]),
+    "pf_dump_wait": (_I, [_H]),
     "pf_kernel_timing": (_I, [_H, _I, C.POINTER(_D), C.POINTER(C.c_int64)]),
     "pf_kernel_timing_reset": (_I, [_H, _I]),
     "pf_checkpoint_size": (_I, [_H, C.POINTER(C.c_size_t)]),
@@ -111,6 +113,8 @@ SIGNATURES = {
     "pf_model_output_due": (_I, [_M, _I, C.POINTER(_I)]),
     "pf_model_write_kinetics": (_I, [_M, C.POINTER(PfStats), _D, _I, C.POINTER(_I)]),
     "pf_write_vti": (_I, [C.c_char_p, _I, _I, _I, _I, _D, C.POINTER(C.c_int32), C.POINTER(_D)]),
+    "pf_write_vti_raw": (_I, [C.c_char_p, _I, _I, _I, _D, C.POINTER(C.c_int32),
+                              C.POINTER(C.c_float)]),
     "pf_write_state": (_I, [C.c_char_p, C.POINTER(PfDesc), _D, C.c_int64,
                             C.POINTER(PfHostState)]),
     "pf_read_state": (_I, [C.c_char_p, C.POINTER(PfDesc), C.POINTER(_D), C.POINTER(C.c_int64),

@@ -12,6 +12,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -75,6 +76,15 @@ struct pf_handle {
   uint8_t *sflag = nullptr;  // per cell: 1 = copied by the last sweep
   bool sweep_hist = false;   // sflag describes the last sweep of the current buffers
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
+  // asynchronous VTI dump: device fields -> pinned host copy (dump stream) -> writer thread
+  void *dump_dev = nullptr;      // korn [N] int32, conc [N] float
+  void *dump_host = nullptr;     // pinned copy of dump_dev
+  hipStream_t dump_stream = nullptr;
+  hipEvent_t dump_ev[2] = {nullptr, nullptr};
+  std::thread dump_thread;
+  bool dump_busy = false;
+  int dump_status = PF_OK;
+  std::string dump_error;
 };
 
 namespace {
@@ -504,6 +514,12 @@ void free_all(pf_handle *h) {
       hipEventDestroy(pr.first);
       hipEventDestroy(pr.second);
     }
+  if (h->dump_thread.joinable()) h->dump_thread.join();
+  if (h->dump_dev) hipFree(h->dump_dev);
+  if (h->dump_host) hipHostFree(h->dump_host);
+  if (h->dump_stream) hipStreamDestroy(h->dump_stream);
+  for (hipEvent_t &ev : h->dump_ev)
+    if (ev) hipEventDestroy(ev);
   if (h->stream) hipStreamDestroy(h->stream);
 }
 
@@ -605,9 +621,74 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
 int pf_destroy(pf_handle *h) {
   if (!h) return PF_OK;
   hipSetDevice(h->d.device);
+  if (h->dump_busy) (void)pf_dump_wait(h);
   if (h->stream) hipStreamSynchronize(h->stream);
   free_all(h);
   delete h;
+  return PF_OK;
+}
+
+int pf_dump_wait(pf_handle *h) {
+  if (!h) return PF_ERR_INVALID;
+  if (!h->dump_busy) return PF_OK;
+  h->dump_thread.join();
+  h->dump_busy = false;
+  if (h->dump_status != PF_OK) return fail(h, h->dump_status, h->dump_error);
+  return PF_OK;
+}
+
+int pf_dump_vti_async(pf_handle *h, const char *path, double dx) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!path) return fail(h, PF_ERR_INVALID, "null path");
+  if ((rc = pf_dump_wait(h)) != PF_OK) return rc;  // the buffers are the previous dump's
+  hipSetDevice(h->d.device);
+  const long N = h->N;
+  if (!h->dump_dev) {
+    HIPCHK(h, hipMalloc(&h->dump_dev, size_t(N) * 8));
+    HIPCHK(h, hipHostMalloc(&h->dump_host, size_t(N) * 8, hipHostMallocDefault));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->dump_stream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreateWithFlags(&h->dump_ev[0], hipEventDisableTiming));
+    HIPCHK(h, hipEventCreateWithFlags(&h->dump_ev[1], hipEventDisableTiming));
+  }
+  int32_t *korn = static_cast<int32_t *>(h->dump_dev);
+  float *conc = reinterpret_cast<float *>(korn + N);
+  // fields of the current state, in stream order with the steps around the call
+  if (is_mixed(h))
+    hipLaunchKernelGGL(vti_fields_kernel<FormMixed>, dim3(blocks_for(N)), dim3(256), 0, h->stream,
+                       h->id[h->cur], h->KI, static_cast<const float *>(h->conc[0][h->ccur[0]]), N,
+                       korn, conc);
+  else
+    hipLaunchKernelGGL(vti_fields_kernel<FormF64>, dim3(blocks_for(N)), dim3(256), 0, h->stream,
+                       h->id[h->cur], h->KI, static_cast<const double *>(h->conc[0][h->ccur[0]]),
+                       N, korn, conc);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->dump_ev[0], h->stream));
+  HIPCHK(h, hipStreamWaitEvent(h->dump_stream, h->dump_ev[0], 0));
+  HIPCHK(h, hipMemcpyAsync(h->dump_host, h->dump_dev, size_t(N) * 8, hipMemcpyDeviceToHost,
+                           h->dump_stream));
+  HIPCHK(h, hipEventRecord(h->dump_ev[1], h->dump_stream));
+  h->dump_busy = true;
+  h->dump_status = PF_OK;
+  h->dump_error.clear();
+  const std::string file(path);
+  const int nx = h->d.nx, ny = h->d.ny, nz = h->d.nz, dev = h->d.device;
+  const hipEvent_t done = h->dump_ev[1];
+  const int32_t *hk = static_cast<const int32_t *>(h->dump_host);
+  const float *hc = reinterpret_cast<const float *>(hk + N);
+  h->dump_thread = std::thread([h, file, nx, ny, nz, dx, dev, done, hk, hc]() {
+    hipSetDevice(dev);
+    const hipError_t e = hipEventSynchronize(done);
+    if (e != hipSuccess) {
+      h->dump_status = PF_ERR_HIP;
+      h->dump_error = std::string("dump copy: ") + hipGetErrorString(e);
+      return;
+    }
+    if (pf_write_vti_raw(file.c_str(), nx, ny, nz, dx, hk, hc) != PF_OK) {
+      h->dump_status = PF_ERR_IO;
+      h->dump_error = "cannot write " + file;
+    }
+  });
   return PF_OK;
 }
 

@@ -134,6 +134,10 @@ __global__ void nucleus_paint_kernel(const long long *__restrict__ center, int q
                                      uint16_t *__restrict__ id, double *__restrict__ phi,
                                      double *__restrict__ pfrac, float *__restrict__ flag,
                                      uint8_t *__restrict__ mark);
+template <class F>
+__global__ void vti_fields_kernel(const uint16_t *__restrict__ id, int KI,
+                                  const typename F::C *__restrict__ c, long ncell,
+                                  int32_t *__restrict__ korn, float *__restrict__ conc);
 __global__ void nucleus_conc_kernel(const uint8_t *__restrict__ mark, double value, long ncell,
                                     double *__restrict__ c);
 __global__ void solute_noise_kernel(double mean, double sd, uint64_t seed_key, long ncell,

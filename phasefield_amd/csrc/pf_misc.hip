@@ -395,6 +395,28 @@ PF_INST_RED(FormMixed, 4, 3)
 PF_INST_RED(FormMixed, 6, 2)
 PF_INST_RED(FormMixed, 6, 3)
 
+// Fields of the VTI dump (write_data.cpp:42-82): korn = 0 where slot 2 holds a grain, else the
+// phase of slot 1 (the grain code's top bits; 0 for an empty cell); conc = the total carbon
+// concentration (mixed form: the stored value minus its compensation, as pf_download_state
+// reports it), as float.
+template <class F>
+__global__ __launch_bounds__(256) void vti_fields_kernel(const uint16_t *__restrict__ id,
+                                                         const int KI,
+                                                         const typename F::C *__restrict__ c,
+                                                         const long ncell,
+                                                         int32_t *__restrict__ korn,
+                                                         float *__restrict__ conc) {
+  const long cell = long(blockIdx.x) * 256 + threadIdx.x;
+  if (cell >= ncell) return;
+  const int g1 = id[cell * KI], g2 = id[cell * KI + 1];
+  korn[cell] = g2 != 0 ? 0 : (g1 >> 14);
+  conc[cell] = float(conc_effective<F>(c, cell, ncell));
+}
+template __global__ void vti_fields_kernel<FormF64>(const uint16_t *, int, const double *, long,
+                                                    int32_t *, float *);
+template __global__ void vti_fields_kernel<FormMixed>(const uint16_t *, int, const float *, long,
+                                                      int32_t *, float *);
+
 #define PF_INST_SETUP(K, P)                                                                      \
   template __global__ void voronoi_kernel<K, P>(const double *, int, double, int, int, int,      \
                                                 long, uint16_t *, double *, double *);           \

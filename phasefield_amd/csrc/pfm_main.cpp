@@ -53,7 +53,17 @@ int write_data(pf_model *m, pf_handle *h, const pf_desc &d, bool forced, int *vt
   if ((rc = pf_model_write_kinetics(m, &st, f_rex, 1, &written)) != PF_OK) return rc;
   int64_t micro = 0;
   pf_model_get_int(m, "is_micro_out", &micro);
-  if (micro) {
+  char fmt[64] = "ascii";  // extension key vti_format=binary: raw VTI, written asynchronously
+  pf_model_get_string(m, "vti_format", fmt, sizeof fmt);
+  if (micro && std::string(fmt) == "binary") {
+    char dir[4096], name[64];
+    pf_model_get_string(m, "directory", dir, sizeof dir);
+    double t = 0, dx = 0;
+    pf_model_get_real(m, "time", &t);
+    pf_model_get_real(m, "dx", &dx);
+    std::snprintf(name, sizeof name, "/field%07d_%d.vti", int(t * 100), (*vti_counter)++);
+    rc = pf_dump_vti_async(h, (std::string(dir) + name).c_str(), dx);
+  } else if (micro) {
     HostState hs(d);
     if ((rc = pf_download_state(h, &hs.view)) != PF_OK) return rc;
     char dir[4096], name[64];
@@ -152,6 +162,7 @@ int main(int argc, char **argv) {
     if (write_data(m, h, d, false, &vti) != PF_OK) return die("output", pf_last_error(h));
   }
   if (pf_synchronize(h) != PF_OK) return die("step", pf_last_error(h));
+  if (pf_dump_wait(h) != PF_OK) return die("output", pf_last_error(h));
   const double wall =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
   std::printf("SIMULATION IS FINISHED.\n\n");

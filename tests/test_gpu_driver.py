@@ -97,3 +97,68 @@ def test_fec_2d_256_1000_steps_bitwise(tmp_path):
         assert np.array_equal(gs[key], os_[key]), key
     # the run did something: austenite grew from its nuclei
     assert gs["phase"].reshape(-1, 3)[:, 0].sum() > s0["phase"].reshape(-1, 3)[:, 0].sum()
+
+
+def _vti_fields(s: dict, d: PfDesc):
+    """korn and float32 carbon of a downloaded state, as write_data.cpp:42-82 defines them."""
+    idx = s["index"].reshape(-1, d.depth)
+    g1, g2 = idx[:, 0], idx[:, 1]
+    ph = np.where(g1 == 0, 0, np.where(g1 // 100000 > 0, g1 // 100000, 1))
+    korn = np.where(g2 > 0, 0, ph).astype(np.int32)
+    N = d.nx * d.ny * d.nz
+    conc = (s["conc"][:N] - s["conc_comp"][:N]).astype(np.float32)
+    return korn, conc
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_async_vti_dump(tmp_path, precision):
+    """pf_dump_vti_async writes the state at the call although the domain keeps stepping while
+    the copy and the file write run; a second dump first waits for the first."""
+    from phasefield_amd import read_vti_raw
+    banner(f"async VTI dump ({precision})")
+    path = configs.write_inputs(str(tmp_path), "fec_3d_256", nx=64, ny=48, nz=40, grains=6,
+                                nuclei=6, precision=precision)
+    pf = PhaseField().read_data(path)
+    pf.construct()
+    pf.nucleate()
+    g = pf.domain
+    d = Model(path).desc()
+    dx = Model(path).get_real("dx")
+    g.step(2)
+    f1, f2 = str(tmp_path / "a.vti"), str(tmp_path / "b.vti")
+    g.dump_vti_async(f1, dx)
+    s1 = g.download()
+    g.step(3)
+    g.dump_vti_async(f2, dx)
+    s2 = g.download()
+    g.step(1)
+    g.dump_wait()
+    for f, s in ((f1, s1), (f2, s2)):
+        r = read_vti_raw(f)
+        korn, conc = _vti_fields(s, d)
+        assert r["shape"] == (d.nx, d.ny, d.nz) and r["dx"] == dx
+        assert np.array_equal(r["korn"], korn), f
+        assert np.array_equal(r["conc"], conc), f
+    assert not np.array_equal(read_vti_raw(f1)["conc"], read_vti_raw(f2)["conc"])
+
+
+def test_pfm_binary_vti_output(tmp_path):
+    """`pfm` with micro output and vti_format=binary: one raw VTI per output time, written
+    asynchronously; the last file holds the final state (also dumped in full precision)."""
+    from phasefield_amd import read_vti_raw
+    banner("pfm binary VTI output")
+    assert os.path.exists(PFM), "phasefield_amd/pfm is not built"
+    dump = str(tmp_path / "state.bin")
+    path = configs.write_inputs(str(tmp_path), "fec_3d_256", nx=48, ny=40, nz=24, grains=4,
+                                nuclei=3, steps=6, output_dt=0.001, micro_out=1,
+                                extra={"vti_format": "binary", "dump_state": dump})
+    r = subprocess.run([PFM, path], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = Model(path).get_string("directory")
+    files = sorted(f for f in os.listdir(out) if f.endswith(".vti"))
+    assert len(files) >= 2
+    d = Model(path).desc()
+    last = max(files, key=lambda f: int(f.rsplit("_", 1)[1][:-4]))
+    v = read_vti_raw(os.path.join(out, last))
+    korn, conc = _vti_fields(_read_dump(dump, d), d)
+    assert np.array_equal(v["korn"], korn) and np.array_equal(v["conc"], conc)

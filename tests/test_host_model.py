@@ -162,6 +162,26 @@ def test_vti_writer(tmp_path):
     assert [float(v) for v in vals] == [float(f"{c:.3g}") for c in conc]
 
 
+def test_vti_raw_writer_roundtrip(tmp_path, rng):
+    """Binary VTI (raw appended): header extent/spacing as the ASCII writer's, arrays back
+    bit for bit, block sizes in the UInt64 headers, and a complete XML tail."""
+    from phasefield_amd import read_vti_raw, write_vti_raw
+    shape = (7, 5, 3)
+    n = 7 * 5 * 3
+    korn = rng.integers(0, 3, n).astype(np.int32)
+    conc = rng.random(n).astype(np.float32)
+    path = str(tmp_path / "f.vti")
+    write_vti_raw(path, shape, 0.1, korn, conc)
+    txt = open(path, "rb").read()
+    assert b'WholeExtent="1 7 1 5 1 3"' in txt and b'header_type="UInt64"' in txt
+    assert b'Name="conc" format="appended" offset="%d"' % (8 + 4 * n) in txt
+    r = read_vti_raw(path)
+    assert r["shape"] == shape and r["dx"] == 0.1
+    assert np.array_equal(r["korn"], korn) and np.array_equal(r["conc"], conc)
+    with pytest.raises(ValueError):
+        write_vti_raw(path, (7, 5, 4), 0.1, korn, conc)
+
+
 def test_state_dump_roundtrip(tmp_path, rng):
     import ctypes as C
     nx, ny, nz, K = 5, 4, 3, 4
