@@ -728,13 +728,17 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       for (int s = nsub - 1; s >= 0; --s) {
         if (split) {
           if (vec && mby == 8)
-            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 8, 2>), dim3(mnb),
-                               dim3(64, 8), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx, v.ny, v.nz,
-                               mzc, mtx, mty, int(mnb));
+            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 8, 2, 1>),
+                               dim3(mnb), dim3(64, 8), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx,
+                               v.ny, v.nz, mzc, mtx, mty, int(mnb));
           else if (vec && mby == 16)
-            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 16, 2>), dim3(mnb),
-                               dim3(64, 16), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx, v.ny, v.nz,
-                               mzc, mtx, mty, int(mnb));
+            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 16, 2, 1>),
+                               dim3(mnb), dim3(64, 16), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx,
+                               v.ny, v.nz, mzc, mtx, mty, int(mnb));
+          else if (vec && mby == 32)
+            hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 32, 2, 2>),
+                               dim3(mnb), dim3(64, 16), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx,
+                               v.ny, v.nz, mzc, mtx, mty, int(mnb));
           else if (vec)
             hipLaunchKernelGGL((diffuse_bulk_vec_kernel<FormF64, P, NF, NOF, 4, 2>),
                                dim3(gvec.nblocks), dim3(64, 4), 0, st, lam_c, lam_c2, sd.flag, dc,

@@ -284,10 +284,11 @@ DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk, int
 // and how long each one marches (at 512^3 the mixed form's sub-step ranges over 357-432 us across
 // tile heights and z-chunks, and the best shape differs between the forms and sizes), which no
 // closed rule captured.  So at the first sub-step of a handle each candidate tile height (8, 16
-// rows) and z-chunk (8..128 planes) is launched and timed into the output buffer, and the fastest
-// is kept.  Every candidate writes the same bits
-// (bulk cells; the flag-0 cells are rewritten by the listed pass), so the choice changes speed
-// only.  PFAMD_DIFF_BY (8 or 16) and PFAMD_DIFF_ZCHUNK pin either choice.
+// rows, one per wavefront) and z-chunk (8..128 planes) is launched and timed into the output
+// buffer, and the fastest is kept.  Every candidate writes the same bits (bulk cells; the flag-0
+// cells are rewritten by the listed pass), so the choice changes speed only.  PFAMD_DIFF_BY (8, 16,
+// or 32: two rows per wavefront, which halves the re-read edge rows but measured 6-7 % slower at
+// 512^3 for its 115 VGPRs) and PFAMD_DIFF_ZCHUNK pin either choice.
 template <class F, int P, int NF, bool NOF>
 hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
                         const typename F::C *cin, typename F::C *cout) {
@@ -298,18 +299,22 @@ hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
     const long nblocks = long(tiles_x) * tiles_y * ((nz + zc - 1) / zc);
     if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     if (by == 8)
-      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 8, V>), dim3(nblocks),
+      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 8, V, 1>), dim3(nblocks),
                          dim3(64, 8), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
                          tiles_x, tiles_y, int(nblocks));
-    else
-      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 16, V>), dim3(nblocks),
+    else if (by == 16)
+      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 16, V, 1>), dim3(nblocks),
+                         dim3(64, 16), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
+                         tiles_x, tiles_y, int(nblocks));
+    else  // 32 rows: two per wavefront
+      hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 32, V, 2>), dim3(nblocks),
                          dim3(64, 16), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
                          tiles_x, tiles_y, int(nblocks));
     return hipGetLastError();
   };
   if (h->march_by == 0) {
     std::vector<int> bys, zcs;
-    if (h->diff_by == 8 || h->diff_by == 16) bys = {h->diff_by};
+    if (h->diff_by == 8 || h->diff_by == 16 || h->diff_by == 32) bys = {h->diff_by};
     else bys = {8, 16};
     if (h->diff_zchunk_override > 0) {
       zcs = {std::min(h->diff_zchunk_override, nz)};
@@ -417,7 +422,7 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
     };
     constexpr int V = F::kMixed ? 4 : 2;  // 16-byte runs per lane
     if (h->diff_split && h->diff_vec && h->diff_march && h->d.nx % V == 0 &&
-        (h->diff_by < 0 || h->diff_by == 8 || h->diff_by == 16)) {
+        (h->diff_by < 0 || h->diff_by == 8 || h->diff_by == 16 || h->diff_by == 32)) {
       const hipError_t e = launch_march<F, P, NF, NOF>(h, dc, cin, cout);
       if (e != hipSuccess) return e;
     } else if (h->diff_split && h->diff_vec && h->d.nx % V == 0 &&

@@ -504,19 +504,6 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
   }
 }
 
-// Plane-marching form of the bulk-only sub-step.  A workgroup owns a tile of BY rows x 64*V cells
-// (one wavefront per row) and walks its z-chunk one plane at a time.  The -z / own / +z runs of
-// each lane stay in registers; the plane being updated is staged in LDS (double-buffered, one
-// barrier per plane) for the y neighbours, and the x neighbours at the ends of a lane's run come
-// from the adjacent lanes (cross-lane permute) or, at the tile's ends, from two wave-uniform
-// loads.  Only the tile's edge rows, its two halo columns and the z-chunk's two end planes are
-// read from global memory a second time, which is what the batched kernel above spends ~30 % of
-// its read traffic on.  Loads are issued in the order they are consumed: this plane's edge rows
-// first, then plane z+2 and the next tags, so the wait before the update covers only loads from
-// the previous iteration and the prefetch stays in flight across the barrier.  Same per-cell
-// arithmetic and face order as diffuse_bulk_vec_kernel (-x, +x, -y, +y, -z, +z), so the two are
-// bitwise interchangeable; cells with flag 0 get a value diffuse_general_kernel overwrites.
-// Needs nx % V == 0; partial tiles (x or y) are masked but keep every barrier.
 // Workgroup barrier ordering LDS only.  __syncthreads() also orders global memory, which makes the
 // compiler wait for every outstanding global load (the next plane's prefetch included) before each
 // barrier; the plane-marching kernel exchanges data between wavefronts through LDS alone.
@@ -526,22 +513,45 @@ __device__ inline void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <class F, int P, int NF, bool NOFLUX, int BY, int V>
-__global__ __launch_bounds__(64 * BY) void diffuse_bulk_march_kernel(const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
-                          const uint8_t *__restrict__ flag, const DiffConsts<typename F::A> dc,
-                          const int nx, const int ny, const int nz, const int zchunk,
-                          const int tiles_x, const int tiles_y, const int nblocks) {
+// Plane-marching form of the bulk-only sub-step.  A workgroup owns a tile of BY rows x 64*V cells,
+// R consecutive rows per wavefront, and walks its z-chunk one plane at a time.  The -z / own / +z
+// runs of each lane stay in registers; the y neighbours come from the lane's own other rows, from
+// the tile's rows staged in LDS (double-buffered, one barrier per plane) or, at the tile's edge
+// rows, from global memory; the x neighbours at the ends of a lane's run are loaded from the cache
+// lines the adjacent lanes read.  Only the tile's edge rows and the z-chunk's two end planes are
+// read from HBM a second time, which is what the batched kernel above spends ~30 % of its read
+// traffic on (2/BY of the rows here).  Loads are issued in the order they are consumed: this
+// plane's edge rows and x ends first, then plane z+2 and its tags, so the wait before the update
+// covers only loads from the previous iteration and the prefetch stays in flight across the
+// barrier.  Same per-cell arithmetic and face order as diffuse_bulk_vec_kernel (-x, +x, -y, +y,
+// -z, +z), so the two are bitwise interchangeable; cells with flag 0 get a value
+// diffuse_general_kernel overwrites.  Needs nx % V == 0; partial tiles (x or y) are masked but
+// keep every barrier.
+template <class F, int P, int NF, bool NOFLUX, int BY, int V, int R>
+__global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
+    const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
+    const uint8_t *__restrict__ flag, const DiffConsts<typename F::A> dc, const int nx,
+    const int ny, const int nz, const int zchunk, const int tiles_x, const int tiles_y,
+    const int nblocks) {
   using A = typename F::A;
   using VA = VecA<A, V>;
-  // per-cell tags of a plane as raw 32-bit words: the compensation codes (mixed, V 16-bit codes
-  // with the flag in their low bits) or the V flag bytes.  Kept packed until the plane is updated
-  // (see the note in the plane step).
+  static_assert(BY % R == 0, "rows per wavefront must divide the tile height");
+  // per-cell tags of a plane as raw words: the compensation codes (mixed, V 16-bit codes with the
+  // flag in their low bits) or the V flag bytes.  Kept packed until the plane is updated (see the
+  // note in the plane step).
   constexpr int kTagWords = F::kMixed ? (2 * V + 3) / 4 : 1;
   using TagWord = std::conditional_t<F::kMixed, uint32_t, uint16_t>;
   struct Tag {
     TagWord w[kTagWords];
   };
   static_assert(F::kMixed ? V == 4 : V == 2, "tag words assume 16-byte runs");
+  // the R rows of one lane in one plane
+  struct Rows {
+    VA c[R];
+  };
+  struct Tags {
+    Tag t[R];
+  };
   __shared__ VA s[2][BY][64];
   const int b = blockIdx.x;
   const int per = nblocks >> 3, rem = nblocks & 7, xcd = b & 7;
@@ -551,118 +561,148 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_march_kernel(const typen
   const int ty_t = t2 % tiles_y;
   const int tz = t2 / tiles_y;
   const int lane = threadIdx.x;
-  const int row = __builtin_amdgcn_readfirstlane(int(threadIdx.y));  // blockDim.x == 64
+  const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.y));  // blockDim.x == 64
   const int tx0 = tx_t * 64 * V;
-  const int x0 = tx0 + lane * V, y = ty_t * BY + row, z0 = tz * zchunk;
-  const bool act = x0 < nx && y < ny;
+  const int x0 = tx0 + lane * V, z0 = tz * zchunk;
   const int z1 = min(z0 + zchunk, nz);
   const long plane = long(nx) * ny;
   const long ncell = plane * nz;
-  const int yc = min(y, ny - 1);           // rows past the domain load a valid row, store nothing
   const int xc = x0 < nx ? x0 : tx0;
-  const int oc = xc + nx * yc;
-  const int oym = xc + nx * wrap(yc - 1, ny), oyp = xc + nx * wrap(yc + 1, ny);
-  // y neighbours from global memory (wave-uniform): first row of the tile, last row of the tile
-  // or of the domain; the others read the tile's rows in LDS
-  const bool ym_g = row == 0, yp_g = row == BY - 1 || y + 1 >= ny;
-  // x neighbours at the ends of the lane's run (4- or 8-byte loads from the cache lines the
-  // adjacent lanes read)
-  const int oxm = wrap(xc - 1, nx) + nx * yc, oxp = wrap(xc + V, nx) + nx * yc;
+  // per row r of the lane: tile row tr = wv*R + r, domain row y (rows past the domain load a
+  // valid row and store nothing)
+  int oc[R], oym[R], oyp[R], oxm[R], oxp[R];
+  bool act[R], ym_g[R], yp_g[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int tr = wv * R + r;
+    const int y = ty_t * BY + tr;
+    const int yc = min(y, ny - 1);
+    act[r] = x0 < nx && y < ny;
+    oc[r] = xc + nx * yc;
+    oym[r] = xc + nx * wrap(yc - 1, ny);
+    oyp[r] = xc + nx * wrap(yc + 1, ny);
+    oxm[r] = wrap(xc - 1, nx) + nx * yc;
+    oxp[r] = wrap(xc + V, nx) + nx * yc;
+    // y neighbours from global memory (wave-uniform): the tile's first row, its last row or the
+    // domain's last row; the others are in the lane's registers or in LDS
+    ym_g[r] = tr == 0;
+    yp_g[r] = tr == BY - 1 || y + 1 >= ny;
+  }
   auto zidx = [&](int z) -> int {
     if (NOFLUX) return z < 0 ? 0 : (z >= nz ? nz - 1 : z);
     return z < 0 ? z + nz : (z >= nz ? z - nz : z);
   };
   auto ldv = [&](int z, int o) -> VA { return *reinterpret_cast<const VA *>(cin + plane * z + o); };
-  auto ldtag = [&](int z) -> Tag {
-    Tag t;
-    if constexpr (F::kMixed) {
-      const uint2 u = *reinterpret_cast<const uint2 *>(
-          reinterpret_cast<const int16_t *>(cin + ncell) + plane * z + oc);
-      t.w[0] = u.x;
-      t.w[1] = u.y;
-    } else {
-      t.w[0] = *reinterpret_cast<const uint16_t *>(flag + plane * z + oc);
+  auto ldrows = [&](int z, Rows &d) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) d.c[r] = ldv(z, oc[r]);
+  };
+  auto ldtags = [&](int z, Tags &d) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if constexpr (F::kMixed) {
+        const uint2 u = *reinterpret_cast<const uint2 *>(
+            reinterpret_cast<const int16_t *>(cin + ncell) + plane * z + oc[r]);
+        d.t[r].w[0] = u.x;
+        d.t[r].w[1] = u.y;
+      } else {
+        d.t[r].w[0] = *reinterpret_cast<const uint16_t *>(flag + plane * z + oc[r]);
+      }
     }
-    return t;
   };
   // Register rings, indexed by the plane's position in the chunk modulo 4: planes z-1, z, z+1 are
   // in use while z+2 is loading, tags of z and z+1 while z+2 is loading.  The loop is unrolled by
   // 4 so that no register still being loaded is ever copied (a copy would wait for the load).
-  VA r0{}, r1 = ldv(z0, oc), r2{}, r3{};
-  if (NF == 6) r0 = ldv(zidx(z0 - 1), oc);
-  r2 = ldv(NF == 6 ? zidx(z0 + 1) : min(z0 + 1, z1 - 1), oc);
-  Tag tg0 = ldtag(z0), tg1{}, tg2{}, tg3{};
-  if (z0 + 1 < z1) tg1 = ldtag(z0 + 1);
-  s[0][row][lane] = r1;
+  Rows r0{}, r1, r2{}, r3{};
+  Tags tg0, tg1{}, tg2{}, tg3{};
+  ldrows(z0, r1);
+  if (NF == 6) ldrows(zidx(z0 - 1), r0);
+  ldrows(NF == 6 ? zidx(z0 + 1) : min(z0 + 1, z1 - 1), r2);
+  ldtags(z0, tg0);
+  if (z0 + 1 < z1) ldtags(z0 + 1, tg1);
+#pragma unroll
+  for (int r = 0; r < R; ++r) s[0][wv * R + r][lane] = r1.c[r];
   lds_barrier();
   // one plane: pm, pc, pp = planes z-1, z, z+1; pn receives plane z+2; tc = tags of plane z, tn
   // receives the tags of plane z+2
-  auto step = [&](const int z, const VA &pm, const VA &pc, const VA &pp, VA &pn, const Tag &tc,
-                  Tag &tn) {
+  auto step = [&](const int z, const Rows &pm, const Rows &pc, const Rows &pp, Rows &pn,
+                  const Tags &tc, Tags &tn) {
     const VA *sp = &s[(z - z0) & 1][0][0];
-    // this plane's values from outside the tile's registers and LDS
-    VA ym, yp;
-    if (ym_g) ym = ldv(z, oym);
-    else ym = sp[(row - 1) * 64 + lane];
-    if (yp_g) yp = ldv(z, oyp);
-    else yp = sp[(row + 1) * 64 + lane];
-    const A xm = A(cin[plane * z + oxm]), xp = A(cin[plane * z + oxp]);
+    // this plane's values from outside the lane's registers and LDS
+    VA ym[R], yp[R];
+    A xm[R], xp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int tr = wv * R + r;
+      if (r > 0) ym[r] = pc.c[r > 0 ? r - 1 : 0];
+      else if (ym_g[r]) ym[r] = ldv(z, oym[r]);
+      else ym[r] = sp[(tr - 1) * 64 + lane];
+      if (yp_g[r]) yp[r] = ldv(z, oyp[r]);
+      else if (r + 1 < R) yp[r] = pc.c[r + 1 < R ? r + 1 : r];
+      else yp[r] = sp[(tr + 1) * 64 + lane];
+      xm[r] = A(cin[plane * z + oxm[r]]);
+      xp[r] = A(cin[plane * z + oxp[r]]);
+    }
     // keep the prefetch below behind these loads: a wait for them then leaves it in flight
     __builtin_amdgcn_sched_barrier(0);
     // then the prefetch: plane z+2 (the +z neighbour of the next plane) and its tags.  Issued on
     // every plane, clamped at the chunk's end to planes already read (cache hits): a load issued
     // on some planes only makes the compiler wait for it inside the branch.
-    pn = ldv(NF == 6 ? zidx(min(z + 2, z1)) : min(z + 2, z1 - 1), oc);
-    tn = ldtag(min(z + 2, z1 - 1));
-    // consume the x loads here, outside the store's branch, so they are not sunk behind the
-    // prefetch (a wait for a load issued after the prefetch would wait for the prefetch too)
-    asm volatile("" ::"v"(xm), "v"(xp));
-    // The empty asm pins the tag words as loaded: without it the compiler unpacks the ring's
-    // tags at the loop's back edge, which waits there for the tags still being prefetched.
-    Tag tg = tc;
+    ldrows(NF == 6 ? zidx(min(z + 2, z1)) : min(z + 2, z1 - 1), pn);
+    ldtags(min(z + 2, z1 - 1), tn);
 #pragma unroll
-    for (int k = 0; k < kTagWords; ++k) asm volatile("" : "+v"(tg.w[k]));
-    VecCode<V> qc{};
-    int fl[V];
+    for (int r = 0; r < R; ++r) {
+      // consume the x loads here, outside the store's branch, so they are not sunk behind the
+      // prefetch (a wait for a load issued after the prefetch would wait for the prefetch too)
+      asm volatile("" ::"v"(xm[r]), "v"(xp[r]));
+      // The empty asm pins the tag words as loaded: without it the compiler unpacks the ring's
+      // tags at the loop's back edge, which waits there for the tags still being prefetched.
+      Tag tg = tc.t[r];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      if constexpr (F::kMixed) {
-        qc.q[v] = int16_t(tg.w[v >> 1] >> (16 * (v & 1)));
-        fl[v] = qc.q[v] & 3;
-      } else {
-        fl[v] = (int(tg.w[0]) >> (8 * v)) & 0xff;
-      }
-    }
-    if (act) {
-      A del[V];
+      for (int k = 0; k < kTagWords; ++k) asm volatile("" : "+v"(tg.w[k]));
+      VecCode<V> qc{};
+      int fl[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const int fv = fl[v];
-        A Dp = dc.D[0];
-#pragma unroll
-        for (int q = 1; q < P; ++q)
-          if (fv == q + 1) Dp = dc.D[q];
-        const A c = pc.v[v];
-        const A l = v == 0 ? xm : pc.v[v > 0 ? v - 1 : 0];
-        const A r = v == V - 1 ? xp : pc.v[v + 1 < V ? v + 1 : v];
-        A d = A(0);
-        d += (l - c) * Dp;
-        d += (r - c) * Dp;
-        if (NF >= 4) {
-          d += (ym.v[v] - c) * Dp;
-          d += (yp.v[v] - c) * Dp;
+        if constexpr (F::kMixed) {
+          qc.q[v] = int16_t(tg.w[v >> 1] >> (16 * (v & 1)));
+          fl[v] = qc.q[v] & 3;
+        } else {
+          fl[v] = (int(tg.w[0]) >> (8 * v)) & 0xff;
         }
-        if (NF == 6) {
-          d += (pm.v[v] - c) * Dp;
-          d += (pp.v[v] - c) * Dp;
-        }
-        del[v] = d * dc.r;
       }
-      vec_update<F, V>(cin, cout, plane * z + oc, ncell, pc, del, qc);
+      if (act[r]) {
+        A del[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int fv = fl[v];
+          A Dp = dc.D[0];
+#pragma unroll
+          for (int q = 1; q < P; ++q)
+            if (fv == q + 1) Dp = dc.D[q];
+          const A c = pc.c[r].v[v];
+          const A l = v == 0 ? xm[r] : pc.c[r].v[v > 0 ? v - 1 : 0];
+          const A rr = v == V - 1 ? xp[r] : pc.c[r].v[v + 1 < V ? v + 1 : v];
+          A d = A(0);
+          d += (l - c) * Dp;
+          d += (rr - c) * Dp;
+          if (NF >= 4) {
+            d += (ym[r].v[v] - c) * Dp;
+            d += (yp[r].v[v] - c) * Dp;
+          }
+          if (NF == 6) {
+            d += (pm.c[r].v[v] - c) * Dp;
+            d += (pp.c[r].v[v] - c) * Dp;
+          }
+          del[v] = d * dc.r;
+        }
+        vec_update<F, V>(cin, cout, plane * z + oc[r], ncell, pc.c[r], del, qc);
+      }
     }
     // plane z+1 into the other buffer (unread after the chunk's last plane); its readers wait at
     // the barrier below, and the buffer read in this plane is only overwritten after the next one
-    s[((z - z0) & 1) ^ 1][row][lane] = pp;
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[((z - z0) & 1) ^ 1][wv * R + r][lane] = pp.c[r];
     lds_barrier();
   };
   int z = z0;
@@ -857,17 +897,18 @@ PF_INST_CLASSIFY(float, 3)
   template __global__ void diffuse_bulk_vec_kernel<F, P, 6, true, 4, V>(                          \
       const F::C *, F::C *, const uint8_t *, const DiffConsts<F::A>, int, int, int, int, int, int, \
       int);
-#define PF_INST_MARCH_ONE(F, P, NF, NOF, BY, V)                                                    \
-  template __global__ void diffuse_bulk_march_kernel<F, P, NF, NOF, BY, V>(                        \
+#define PF_INST_MARCH_ONE(F, P, NF, NOF, BY, V, R)                                                 \
+  template __global__ void diffuse_bulk_march_kernel<F, P, NF, NOF, BY, V, R>(                     \
       const F::C *, F::C *, const uint8_t *, const DiffConsts<F::A>, int, int, int, int, int, int, \
       int);
-#define PF_INST_MARCH(F, P, V)                 \
-  PF_INST_MARCH_ONE(F, P, 4, false, 8, V)      \
-  PF_INST_MARCH_ONE(F, P, 6, false, 8, V)      \
-  PF_INST_MARCH_ONE(F, P, 6, true, 8, V)       \
-  PF_INST_MARCH_ONE(F, P, 4, false, 16, V)     \
-  PF_INST_MARCH_ONE(F, P, 6, false, 16, V)     \
-  PF_INST_MARCH_ONE(F, P, 6, true, 16, V)
+#define PF_INST_MARCH_NF(F, P, BY, V, R)          \
+  PF_INST_MARCH_ONE(F, P, 4, false, BY, V, R)     \
+  PF_INST_MARCH_ONE(F, P, 6, false, BY, V, R)     \
+  PF_INST_MARCH_ONE(F, P, 6, true, BY, V, R)
+#define PF_INST_MARCH(F, P, V)          \
+  PF_INST_MARCH_NF(F, P, 8, V, 1)       \
+  PF_INST_MARCH_NF(F, P, 16, V, 1)      \
+  PF_INST_MARCH_NF(F, P, 32, V, 2)
 PF_INST_MARCH(FormF64, 2, 2)
 PF_INST_MARCH(FormF64, 3, 2)
 PF_INST_MARCH(FormMixed, 2, 4)

@@ -58,7 +58,7 @@ __global__ void diffuse_bulk_vec_kernel(const typename F::C *__restrict__ cin,
                                         const int tiles_x, const int tiles_y, const int nblocks);
 
 // plane-marching bulk kernel: same contract as diffuse_bulk_vec_kernel
-template <class F, int P, int NF, bool NOFLUX, int BY, int V>
+template <class F, int P, int NF, bool NOFLUX, int BY, int V, int R>
 __global__ void diffuse_bulk_march_kernel(const typename F::C *__restrict__ cin,
                                           typename F::C *__restrict__ cout,
                                           const uint8_t *__restrict__ flag,

@@ -124,6 +124,7 @@ def test_full_steps_bitwise(tmp_path, monkeypatch, shape, bc, K, sweep_split):
 @pytest.mark.parametrize("kernel,rows,split,vec,march,zchunk", [
     ("bulk", "-1", "1", "1", "1", "0"),
     ("bulk", "8", "1", "1", "1", "5"), ("bulk", "16", "1", "1", "1", "3"),
+    ("bulk", "32", "1", "1", "1", "7"),
     ("bulk", "-1", "1", "1", "0", "0"),
     ("bulk", "0", "1", "0", "0", "0"), ("bulk", "4", "1", "0", "0", "0"),
     ("bulk", "8", "1", "0", "0", "0"), ("bulk", "16", "1", "0", "0", "0"),
@@ -134,7 +135,8 @@ def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, 
                                            zchunk, form):
     """Every diffusion kernel against the oracle on a seeded microstructure (bulk and interface
     cells), in both forms: the plane-marching bulk kernel (timed choice of tile and z-chunk, and
-    8/16-row tiles with 5/3-plane chunks), the batched 16-byte-run bulk kernel, storage order and
+    8/16-row tiles with 5/3-plane chunks, 32-row tiles with two rows per wavefront and 7-plane
+    chunks), the batched 16-byte-run bulk kernel, storage order and
     column-marching with 4/8/16-row tiles (full-formula cells inline or in their own listed
     pass), and the LDS-tiled full formula."""
     banner(f"diffusion variant {kernel} rows={rows} split={split} vec={vec} march={march} "
@@ -153,7 +155,7 @@ def test_diffusion_kernel_variants_bitwise(tmp_path, monkeypatch, kernel, rows, 
 
 
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
-@pytest.mark.parametrize("rows,zchunk", [("8", "5"), ("16", "4"), ("-1", "0")])
+@pytest.mark.parametrize("rows,zchunk", [("8", "5"), ("16", "4"), ("32", "6"), ("-1", "0")])
 def test_march_noflux_bitwise(tmp_path, monkeypatch, rows, zchunk, form):
     """The plane-marching bulk kernel with no-flux z faces: chunk end planes clamp to the cell
     itself at the domain's z ends and read the neighbour chunk's plane inside."""
