@@ -57,9 +57,9 @@ def setup_fp64(m, p):
 def alg_bytes_per_substep(mixed: bool, nphase: int) -> int:
     """Compulsory bytes of one carbon diffusion sub-step per cell: read value_phase (P values) and
     the stored concentration, write the concentration (FP64: 8-byte values; mixed: 4-byte phase
-    fractions and a 6-byte concentration = FP32 value + 16-bit Kahan compensation code)."""
+    fractions and a 5-byte concentration = FP32 value + one-byte Kahan compensation code)."""
     if mixed:
-        return nphase * 4 + 6 + 6
+        return nphase * 4 + 5 + 5
     return nphase * 8 + 8 + 8
 
 

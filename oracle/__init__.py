@@ -68,6 +68,8 @@ def lib():
                                             P(_D), P(OrcParams), P(_D), P(_D)]),
             "orc_mt19937_nth": (C.c_uint32, [C.c_uint32, _I]),
             "orc_splitmix64": (C.c_uint64, [C.c_uint64]),
+            "orc_quantize_comp": (C.c_float, [C.c_float, C.c_float, C.c_uint32]),
+            "orc_mix32": (C.c_uint32, [C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

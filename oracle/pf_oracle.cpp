@@ -18,7 +18,7 @@
 //
 // Three number forms share the code: FP64 (the reference), MIXED (FP32 storage and arithmetic,
 // FP64 for the clamp/prune decision and renormalisation, Kahan-compensated concentration update
-// with the compensation held as a 14-bit fixed-point fraction of an ulp)
+// with the compensation held as an 8-bit fixed-point fraction of an ulp, rounded stochastically)
 // and DUAL (FP64 with forward-mode derivatives, the reference gradient for the adjoint).
 
 #include "pf_oracle.h"
@@ -93,17 +93,28 @@ template <> inline double as<double, Dual>(const Dual &x) { return x.v; }
 
 using std::sqrt;
 
-// Mixed form: the Kahan compensation is kept as a 14-bit signed fixed-point multiple of 2^-13 ulp
-// of the stored value (comp = q * 2^(e - 37), value = m * 2^e with 0.5 <= |m| < 1, q saturating at
-// [-8192, 8191]); this returns the compensation that representation holds.
-inline float quantize_comp(float comp, float value) {
+// Mixed form: the Kahan compensation is kept as an 8-bit signed fixed-point multiple of 2^-7 ulp
+// of the stored value (comp = q * 2^(e - 31), value = m * 2^e with 0.5 <= |m| < 1, q saturating at
+// [-128, 127]), rounded stochastically: q = floor(x + u) for the scaled value x and a threshold u
+// in [0, 1) with 16 fraction bits hashed from the cell index and the sub-step (key).  This returns
+// the compensation that representation holds.
+inline uint32_t mix32(uint32_t x) {  // xorshift-multiply integer hash ("lowbias32" form)
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+inline float quantize_comp(float comp, float value, uint32_t key) {
   int e = 0;
   (void)std::frexp(value, &e);
-  float q = std::rint(std::ldexp(comp, 37 - e));
-  q = std::fmin(std::fmax(q, -8192.f), 8191.f);
-  return std::ldexp(q, e - 37);
+  const float x = std::ldexp(comp, 31 - e);
+  float q = std::floor(x + float(key >> 16) * 0x1p-16f);
+  q = std::fmin(std::fmax(q, -128.f), 127.f);
+  return std::ldexp(q, e - 31);
 }
-template <class T> inline T quantize_comp(T comp, T) { return comp; }
+template <class T> inline T quantize_comp(T comp, T, uint32_t) { return comp; }
 
 template <class S, class A, class D, bool K> struct Form {
   using Store = S;  // phi, value_phase, concentrations
@@ -158,6 +169,7 @@ template <class F> struct Domain : DomainBase {
   std::vector<int32_t> idx1, idx2;
   std::vector<S> val1, val2, vph;
   std::vector<S> conc[2], comp[2], vc1[2], vc2[2];
+  uint32_t substeps[2] = {0, 0};  // sub-steps per solute since the state was set (rounding keys)
   // 1-based phase tables; row/column 0 belong to "no phase" (empty slot) and are kept finite
   A M[4][4], sig[4][4], ent[4][4], eqc[4][4], slope[4][4];
   S pk[2][4][4], pc[2][4][4];
@@ -251,11 +263,13 @@ template <class F> struct Domain : DomainBase {
       val2[n] = S(0.0);
     }
     for (long n = 0; n < N * 3; ++n) vph[n] = cvt<S>(phase[n]);
-    for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NS; ++s) {
+      substeps[s] = 0;
       for (long n = 0; n < N; ++n) {
         conc[s][n] = cvt<S>(c[s * N + n]);
         comp[s][n] = S(0.0);
       }
+    }
     repartition();
   }
 
@@ -400,6 +414,8 @@ template <class F> struct Domain : DomainBase {
   void diffuse(int s, int nsteps, double factor) override {
     const A r = cvt<A>(factor);
     for (int it = 0; it < nsteps; ++it) {
+      // the sub-step's rounding key: mix32(k * 2 + s) for the k-th sub-step of solute s
+      const uint32_t key = mix32(substeps[s]++ * 2u + uint32_t(s));
 #pragma omp parallel for schedule(static) collapse(2)
       for (int k = 0; k < nz; ++k)
         for (int j = 0; j < ny; ++j)
@@ -416,7 +432,7 @@ template <class F> struct Domain : DomainBase {
             if (F::kahan) {
               const S y = S(del) - comp[s][c];
               const S t = conc[s][c] + y;
-              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t);
+              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t, mix32(uint32_t(c) ^ key));
               conc[s][c] = t;
             } else {
               conc[s][c] += S(del);
@@ -565,13 +581,15 @@ template <class F> struct Domain : DomainBase {
       std::swap(val1, val2);
     }
     // total_conc = dot(value_phase, value_conc1); Partitioning (MicrostructureConstruction.cpp:54-62)
-    for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NS; ++s) {
+      substeps[s] = 0;
       for (long n = 0; n < N; ++n) {
         S t = S(0.0);
         for (int q = 0; q < 3; ++q) t += vph[n * 3 + q] * vc1[s][n * 3 + q];
         conc[s][n] = t;
         comp[s][n] = S(0.0);
       }
+    }
     repartition();
   }
 
@@ -847,5 +865,9 @@ uint32_t orc_mt19937_nth(uint32_t seed, int n) {
   return v;
 }
 uint64_t orc_splitmix64(uint64_t x) { return splitmix64(x); }
+float orc_quantize_comp(float comp, float value, uint32_t key) {
+  return quantize_comp(comp, value, key);
+}
+uint32_t orc_mix32(uint32_t x) { return mix32(x); }
 
 }  // extern "C"

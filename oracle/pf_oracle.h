@@ -86,6 +86,10 @@ void orc_update_parameter(double temperature, double dx, double dims, const doub
                           double *diff_dt);
 uint32_t orc_mt19937_nth(uint32_t seed, int n);
 uint64_t orc_splitmix64(uint64_t x);
+// the mixed form's stored compensation of comp for a cell holding value, with rounding key key
+// (stochastic rounding to a multiple of 2^-7 ulp(value)), and the integer hash the keys come from
+float orc_quantize_comp(float comp, float value, uint32_t key);
+uint32_t orc_mix32(uint32_t x);
 
 #ifdef __cplusplus
 }

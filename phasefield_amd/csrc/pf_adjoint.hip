@@ -555,6 +555,7 @@ struct StepData {
   DevState in;
   double *pf_new = nullptr;
   uint8_t *flag = nullptr;
+  unsigned *count = nullptr;  // [4] bulk cells per phase of that value_phase
   double *csub = nullptr;
 };
 
@@ -612,6 +613,7 @@ template <int K, int P, int NF, bool NOF> struct Adj {
     for (int q = 0; q < nstore; ++q) {
       seg[q].pf_new = ar.get<double>(size_t(N) * P);
       seg[q].flag = ar.get<uint8_t>(N);
+      seg[q].count = ar.get<unsigned>(4);
       seg[q].csub = ar.get<double>(size_t(N) * (nsub + 1));
     }
     if (ar.err != hipSuccess)
@@ -695,7 +697,7 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       const DevState cur = on_handle();
       hipMemcpyAsync(sd.pf_new, cur.pf, n8 * P, hipMemcpyDeviceToDevice, st);
       (void)launch_classify<double, P, NF, NOF>(st, sd.pf_new, unit_tmp, sd.flag, v.nx, v.ny,
-                                                 v.nz, N, v.bulk_mask, nullptr, nullptr);
+                                                 v.nz, N, v.bulk_mask, sd.count);
       hipMemcpyAsync(sd.csub, cur.c, n8, hipMemcpyDeviceToDevice, st);
       for (int s = 0; s < nsub && r == PF_OK; ++s)
         r = handle_diffuse_io(h, sd.csub + size_t(N) * s, sd.csub + size_t(N) * (s + 1));
@@ -712,32 +714,35 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         const double g = 2.0 * (frac[qs] - sp->target_fraction[qs]) / double(N);
         hipLaunchKernelGGL(adj_seed_fraction_kernel, dim3(gb), dim3(256), 0, st, lam_pf, P, g, N);
       }
-      if (split) {
-        // the listed cells of this step: interface cells and bulk cells of phase 1
-        hipLaunchKernelGGL(adj_list_flag_kernel, dim3(gb), dim3(256), 0, st, sd.flag, N);
-        hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, st, sd.flag, N, ablock);
-        hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl));
-        hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, N, ablock,
-                           alist);
-      }
       // the forward step's plane-marching shape, when its first sub-step has picked one
       int mby = 0, mzc = 0;
       handle_march_shape(h, &mby, &mzc);
+      if (split) {
+        // the listed cells of this step: interface cells and bulk cells of phase 1, and, when the
+        // plane-marching kernel (one bulk phase for every cell) runs, every cell not of its phase
+        const unsigned *keep = vec && mby ? sd.count : nullptr;
+        hipLaunchKernelGGL(adj_list_flag_kernel, dim3(gb), dim3(256), 0, st, sd.flag, N);
+        hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
+                           ablock);
+        hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl));
+        hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
+                           ablock, alist);
+      }
       const int mtx = (v.nx + 127) / 128, mty = mby ? (v.ny + mby - 1) / mby : 0;
       const long mnb = mby ? long(mtx) * mty * ((v.nz + mzc - 1) / mzc) : 0;
       for (int s = nsub - 1; s >= 0; --s) {
         if (split) {
           if (vec && mby == 8)
             hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 8, 2, 1>),
-                               dim3(mnb), dim3(64, 8), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx,
+                               dim3(mnb), dim3(64, 8), 0, st, lam_c, lam_c2, sd.count, dc, v.nx,
                                v.ny, v.nz, mzc, mtx, mty, int(mnb));
           else if (vec && mby == 16)
             hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 16, 2, 1>),
-                               dim3(mnb), dim3(64, 16), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx,
+                               dim3(mnb), dim3(64, 16), 0, st, lam_c, lam_c2, sd.count, dc, v.nx,
                                v.ny, v.nz, mzc, mtx, mty, int(mnb));
           else if (vec && mby == 32)
             hipLaunchKernelGGL((diffuse_bulk_march_kernel<FormF64, P, NF, NOF, 32, 2, 2>),
-                               dim3(mnb), dim3(64, 16), 0, st, lam_c, lam_c2, sd.flag, dc, v.nx,
+                               dim3(mnb), dim3(64, 16), 0, st, lam_c, lam_c2, sd.count, dc, v.nx,
                                v.ny, v.nz, mzc, mtx, mty, int(mnb));
           else if (vec)
             hipLaunchKernelGGL((diffuse_bulk_vec_kernel<FormF64, P, NF, NOF, 4, 2>),

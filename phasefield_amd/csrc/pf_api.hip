@@ -45,6 +45,8 @@ struct pf_handle {
   void *conc[PF_MAX_SOLUTE][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   int cur = 0;                       // active slot buffer
   int ccur[PF_MAX_SOLUTE] = {0, 0};  // active concentration buffer per solute
+  // sub-steps applied per solute since the state was set: the mixed form's rounding keys
+  uint32_t csub[PF_MAX_SOLUTE] = {0, 0};
   RedPartial *red = nullptr;
   int red_blocks = 0;
   pf_params p{};
@@ -62,6 +64,7 @@ struct pf_handle {
   int diff_variant = 0;  // 0: bulk-aware column kernel, 1: LDS-tiled full-formula kernel
   uint8_t *flag = nullptr;   // per-cell bulk phase (0 = needs the full formula)
   uint8_t *unit = nullptr;   // scratch of the classification: per-cell unit-vector phase
+  unsigned *bulk_count = nullptr;  // [4] bulk cells per phase of the last classification
   bool flag_valid = false;   // flags describe the current value_phase and stencil
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
@@ -214,11 +217,11 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     hipLaunchKernelGGL((sweep_classify_kernel<F, K, NF, NOF>), dim3(nb), dim3(256), 0, h->stream,
                        h->id[a], phi1, h->id[b], phi2, h->sflag, h->sweep_hist ? 1 : 0,
                        h->p.cutoff, h->d.nx, h->d.ny, h->d.nz, h->N);
-    hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag, h->N,
-                       h->gblock);
+    hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
+                       static_cast<const unsigned *>(nullptr), h->N, h->gblock);
     hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock, int(nl));
-    hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag, h->N,
-                       h->gblock, h->glist);
+    hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
+                       static_cast<const unsigned *>(nullptr), h->N, h->gblock, h->glist);
     // grid-stride over the list: enough workgroups to fill every CU at the LDS-limited occupancy
     const unsigned ng = std::min<unsigned>(blocks_for(h->N, kSweepBS), 4096);
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true>), dim3(ng), dim3(kSweepBS),
@@ -300,15 +303,15 @@ hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
     if (nblocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     if (by == 8)
       hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 8, V, 1>), dim3(nblocks),
-                         dim3(64, 8), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
+                         dim3(64, 8), 0, h->stream, cin, cout, h->bulk_count, dc, nx, ny, nz, zc,
                          tiles_x, tiles_y, int(nblocks));
     else if (by == 16)
       hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 16, V, 1>), dim3(nblocks),
-                         dim3(64, 16), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
+                         dim3(64, 16), 0, h->stream, cin, cout, h->bulk_count, dc, nx, ny, nz, zc,
                          tiles_x, tiles_y, int(nblocks));
     else  // 32 rows: two per wavefront
       hipLaunchKernelGGL((diffuse_bulk_march_kernel<F, P, NF, NOF, 32, V, 2>), dim3(nblocks),
-                         dim3(64, 16), 0, h->stream, cin, cout, h->flag, dc, nx, ny, nz, zc,
+                         dim3(64, 16), 0, h->stream, cin, cout, h->bulk_count, dc, nx, ny, nz, zc,
                          tiles_x, tiles_y, int(nblocks));
     return hipGetLastError();
   };
@@ -376,23 +379,25 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
                        h->d.nz, g.zchunk, g.tiles_x, g.tiles_y, g.nblocks);
   } else {
     const unsigned nb = blocks_for(h->N);
+    constexpr int V = F::kMixed ? 4 : 2;  // 16-byte runs per lane
+    const bool march = h->diff_split && h->diff_vec && h->diff_march && h->d.nx % V == 0 &&
+                       (h->diff_by < 0 || h->diff_by == 8 || h->diff_by == 16 || h->diff_by == 32);
     if (!h->flag_valid || h->flag_nf != NF) {
-      int16_t *codes[2] = {nullptr, nullptr};
-      if (F::kMixed)
-        for (int q = 0; q < h->d.nsolute; ++q)
-          codes[q] = comp_codes(static_cast<float *>(h->conc[q][h->ccur[q]]), h->N);
       const hipError_t ec =
           launch_classify<T, P, NF, NOF>(h->stream, pf, h->unit, h->flag, h->d.nx, h->d.ny,
-                                         h->d.nz, h->N, h->bulk_mask, codes[0], codes[1]);
+                                         h->d.nz, h->N, h->bulk_mask, h->bulk_count);
       if (ec != hipSuccess) return ec;
       if (h->diff_split) {
+        // the plane-marching kernel leaves every cell whose flag is not its bulk phase; the other
+        // bulk kernels leave the flag-0 cells
+        const unsigned *keep = march ? h->bulk_count : nullptr;
         const unsigned nl = list_blocks(h->N);
-        hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, h->N,
-                           h->gblock);
+        hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, keep,
+                           h->N, h->gblock);
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock,
                            int(nl));
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag,
-                           h->N, h->gblock, h->glist);
+                           keep, h->N, h->gblock, h->glist);
       }
       h->flag_valid = true;
       h->flag_nf = NF;
@@ -420,9 +425,7 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
       else
         go(std::integral_constant<int, 4>(), split);
     };
-    constexpr int V = F::kMixed ? 4 : 2;  // 16-byte runs per lane
-    if (h->diff_split && h->diff_vec && h->diff_march && h->d.nx % V == 0 &&
-        (h->diff_by < 0 || h->diff_by == 8 || h->diff_by == 16 || h->diff_by == 32)) {
+    if (march) {
       const hipError_t e = launch_march<F, P, NF, NOF>(h, dc, cin, cout);
       if (e != hipSuccess) return e;
     } else if (h->diff_split && h->diff_vec && h->d.nx % V == 0 &&
@@ -466,14 +469,16 @@ hipError_t dispatch_diffuse_p(pf_handle *h, int s, const DiffConsts<typename F::
 
 template <class F>
 int run_diffuse(pf_handle *h, int s, int n, double factor) {
-  const auto dc = make_diff_consts<typename F::A>(h->p, s, factor);
+  auto dc = make_diff_consts<typename F::A>(h->p, s, factor);
   if (n <= 0) return PF_OK;
   // the first launch classifies bulk cells when the flags are stale; keep it out of the timing
+  dc.key = comp_step_key(h->csub[s]++, s);
   hipError_t e = h->d.nphase == 2 ? dispatch_diffuse_p<F, 2>(h, s, dc)
                                   : dispatch_diffuse_p<F, 3>(h, s, dc);
   if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("diffuse: ") + hipGetErrorString(e));
   Timer t(h, 0, n - 1);
   for (int it = 1; it < n; ++it) {
+    dc.key = comp_step_key(h->csub[s]++, s);
     e = h->d.nphase == 2 ? dispatch_diffuse_p<F, 2>(h, s, dc) : dispatch_diffuse_p<F, 3>(h, s, dc);
     if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("diffuse: ") + hipGetErrorString(e));
   }
@@ -510,6 +515,7 @@ void free_all(pf_handle *h) {
   if (h->pfrac) hipFree(h->pfrac);
   if (h->flag) hipFree(h->flag);
   if (h->unit) hipFree(h->unit);
+  if (h->bulk_count) hipFree(h->bulk_count);
   if (h->glist) hipFree(h->glist);
   if (h->sflag) hipFree(h->sflag);
   if (h->gblock) hipFree(h->gblock);
@@ -574,7 +580,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   h->dims = dims;
   h->NF = 2 * dims;
   h->tsz = d.precision == PF_MIXED ? 4 : 8;
-  h->csz = d.precision == PF_MIXED ? 6 : 8;  // double, or float value + int16 compensation code
+  h->csz = d.precision == PF_MIXED ? 5 : 8;  // double, or float value + one-byte compensation code
   if (const char *e = std::getenv("PFAMD_DIFF_ZCHUNK")) h->diff_zchunk_override = std::atoi(e);
   if (const char *e = std::getenv("PFAMD_DIFF_KERNEL"))
     h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
@@ -604,6 +610,8 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     return rc;
   if ((rc = chk(hipMalloc(&h->flag, N), "alloc bulk flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->unit, N), "alloc bulk flags")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->bulk_count, 4 * sizeof(unsigned)), "alloc bulk counts")) != PF_OK)
+    return rc;
   if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->sflag, N), "alloc sweep copy flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->gblock, (list_blocks(h->N) + 1) * sizeof(int)),
@@ -760,11 +768,11 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
     HIPCHK(h, hipMemcpyAsync(h->pfrac, ph.data(), ph.size() * 4, hipMemcpyHostToDevice, h->stream));
     for (int sol = 0; sol < h->d.nsolute; ++sol) {
       // [value][compensation code] per buffer
-      std::vector<float> c(size_t(N) + (size_t(N) + 1) / 2);
-      int16_t *q = comp_codes(c.data(), N);
+      std::vector<float> c(size_t(N) + (size_t(N) + 3) / 4);
+      CompCode *q = comp_codes(c.data(), N);
       for (long n = 0; n < N; ++n) {
         c[n] = float(s->conc[sol * N + n]);
-        q[n] = s->conc_comp ? comp_encode(float(s->conc_comp[sol * N + n]), c[n], 0) : int16_t(0);
+        q[n] = s->conc_comp ? comp_encode(float(s->conc_comp[sol * N + n]), c[n], 0u) : CompCode(0);
       }
       HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), N * h->csz, hipMemcpyHostToDevice,
                                h->stream));
@@ -784,6 +792,7 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
   }
   h->cur = 0;
   h->ccur[0] = h->ccur[1] = 0;
+  h->csub[0] = h->csub[1] = 0;
   h->have_state = true;
   h->flag_valid = false;
   h->sweep_hist = false;
@@ -825,10 +834,10 @@ int pf_download_state(pf_handle *h, pf_host_state *s) {
     for (long n = 0; n < N; ++n)
       for (int p = 0; p < P; ++p) phase[size_t(n) * 3 + p] = ph[size_t(n) * P + p];
     for (int sol = 0; sol < h->d.nsolute; ++sol) {
-      std::vector<float> c(size_t(N) + (size_t(N) + 1) / 2);
+      std::vector<float> c(size_t(N) + (size_t(N) + 3) / 4);
       HIPCHK(h, hipMemcpy(c.data(), h->conc[sol][h->ccur[sol]], N * h->csz,
                           hipMemcpyDeviceToHost));
-      const int16_t *q = comp_codes(c.data(), N);
+      const CompCode *q = comp_codes(c.data(), N);
       for (long n = 0; n < N; ++n) {
         conc[sol * N + n] = c[n];
         comp[sol * N + n] = comp_decode(q[n], c[n]);
@@ -1028,9 +1037,11 @@ int pf_checkpoint_save(pf_handle *h, void *dst) {
   hipSetDevice(h->d.device);
   const size_t N = size_t(h->N);
   char *p = static_cast<char *>(dst);
+  static_assert(8 + sizeof(pf_desc) + sizeof(h->csub) <= 64, "checkpoint header");
   std::memset(p, 0, 64);
-  std::memcpy(p, "PFCKPT01", 8);
+  std::memcpy(p, "PFCKPT02", 8);
   std::memcpy(p + 8, &h->d, sizeof(pf_desc));
+  std::memcpy(p + 8 + sizeof(pf_desc), h->csub, sizeof(h->csub));
   p += 64;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(p, h->id[h->cur], N * h->KI * 2, hipMemcpyDeviceToHost));
@@ -1053,8 +1064,10 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
   const char *p = static_cast<const char *>(src);
   pf_desc d;
   std::memcpy(&d, p + 8, sizeof(d));
-  if (std::memcmp(p, "PFCKPT01", 8) != 0 || std::memcmp(&d, &h->d, sizeof(d)) != 0)
+  if (std::memcmp(p, "PFCKPT02", 8) != 0 || std::memcmp(&d, &h->d, sizeof(d)) != 0)
     return fail(h, PF_ERR_INVALID, "checkpoint does not match this domain");
+  uint32_t csub[PF_MAX_SOLUTE];
+  std::memcpy(csub, p + 8 + sizeof(pf_desc), sizeof(csub));
   p += 64;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(h->id[h->cur], p, N * h->KI * 2, hipMemcpyHostToDevice));
@@ -1067,6 +1080,7 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
     HIPCHK(h, hipMemcpy(h->conc[s][h->ccur[s]], p, N * h->csz, hipMemcpyHostToDevice));
     p += N * h->csz;
   }
+  std::memcpy(h->csub, csub, sizeof(csub));
   h->have_state = true;
   h->flag_valid = false;
   h->sweep_hist = false;
@@ -1117,6 +1131,7 @@ int pf_setup_construct(pf_handle *h, const double *seeds, int G, double reductio
   }
   for (int s = 0; s < h->d.nsolute; ++s) {
     h->ccur[s] = 0;
+    h->csub[s] = 0;
     double *c = static_cast<double *>(h->conc[s][0]);
     if (P == 2)
       hipLaunchKernelGGL((construct_conc_kernel<2>), dim3(blocks_for(h->N)), dim3(256), 0,
@@ -1262,6 +1277,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
                        static_cast<float *>(dst->pfrac), long(N) * a.nphase);
     for (int s = 0; s < a.nsolute; ++s) {
       dst->ccur[s] = 0;
+      dst->csub[s] = 0;
       hipLaunchKernelGGL(convert_conc_f64_to_mixed, dim3(blocks_for(long(N))), dim3(256), 0,
                          dst->stream, static_cast<const double *>(src->conc[s][src->ccur[s]]),
                          static_cast<float *>(dst->conc[s][0]), long(N));
@@ -1274,6 +1290,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
                                dst->stream));
     for (int s = 0; s < a.nsolute; ++s) {
       dst->ccur[s] = 0;
+      dst->csub[s] = 0;
       HIPCHK(dst, hipMemcpyAsync(dst->conc[s][0], src->conc[s][src->ccur[s]], N * 8,
                                  hipMemcpyDeviceToDevice, dst->stream));
     }

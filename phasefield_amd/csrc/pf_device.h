@@ -5,8 +5,8 @@
 //   slot phi    T      [N][K]    double (FP64 form) or float (mixed form), double-buffered
 //   phase frac  T      [N][P]    value_phase, P = 2 or 3 phases carried
 //   conc        C      [S][N]    per solute, double-buffered (Jacobi sub-steps):
-//                                FP64: double [N];  mixed: float value [N] followed by the int16
-//                                Kahan compensation code [N] in the same buffer (6 B per cell)
+//                                FP64: double [N];  mixed: float value [N] followed by the one-byte
+//                                Kahan compensation code [N] in the same buffer (5 B per cell)
 // A slot id is a 16-bit grain code: bits 15..14 = phase (1..3), bits 13..0 = grain serial;
 // 0 = empty slot.  The host maps the reference's int32 grain ids (100000+n, 200000+g) to codes
 // and back; the step only ever compares codes for equality, so results do not depend on it.
@@ -37,25 +37,48 @@ struct FormMixed {
 
 template <class C> __host__ __device__ inline C conc_hi(C c) { return c; }
 
-// Mixed-form Kahan compensation, stored in a 16-bit code with the cell's bulk flag:
-// code = q * 4 + flag, comp = q * 2^(e - 37) where c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. a
-// 14-bit signed multiple of 2^-13 ulp(c) (ulp(c) = 2^(e - 24)).  |comp| <= ulp/2 after a Kahan
-// update, so |q| <= 2^12 with 2x headroom; q saturates at [-8192, 8191].  The effective
-// concentration is c - comp.  The two flag bits (the bulk phase of the cell, 0 = full formula)
-// are set by the per-sweep classification and carried unchanged by every sub-step.
-constexpr int kCompShift = 37;
+// Mixed-form Kahan compensation, stored in one signed byte per cell: comp = q * 2^(e - 31) where
+// c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. a multiple of 2^-7 ulp(c) (ulp(c) = 2^(e - 24)).
+// |comp| <= ulp/2 after a Kahan update, so |q| <= 64 with 2x headroom; q saturates at [-128, 127].
+// The effective concentration is c - comp.
+//
+// q is rounded stochastically: down or up to a neighbouring multiple with probability given by the
+// distance, from a 16-bit threshold u hashed from the cell index and the sub-step.  Rounding to
+// nearest would drop every increment below half a quantum (an increment too small to move c goes
+// into comp and is rounded away again, sub-step after sub-step), and those losses do not cancel
+// between neighbours of different exponents; with the unbiased rounding they average out.  x + u is
+// exact for |x| < 256 (u has 16 fraction bits), so a comp already on the grid stays unchanged.
+using CompCode = int8_t;
+constexpr int kCompShift = 31;
 __host__ __device__ inline int comp_exp(float c) {
   int e = 0;
   (void)frexpf(c, &e);
   return e;
 }
-__host__ __device__ inline float comp_decode(int16_t code, float c) {
-  return ldexpf(float(int(code) >> 2), comp_exp(c) - kCompShift);
+__host__ __device__ inline float comp_decode(CompCode code, float c) {
+  return ldexpf(float(code), comp_exp(c) - kCompShift);
 }
-__host__ __device__ inline int16_t comp_encode(float comp, float c, int flag_bits) {
-  float v = rintf(ldexpf(comp, kCompShift - comp_exp(c)));
-  v = fminf(fmaxf(v, -8192.f), 8191.f);
-  return int16_t(int(v) * 4 + (flag_bits & 3));
+// 32-bit integer mixer (xorshift-multiply rounds of the "lowbias32" form)
+__host__ __device__ inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+// key of one sub-step of solute s: k counts the sub-steps applied to that solute since the state
+// was set; the rounding threshold of cell i is then taken from mix32(i ^ key)
+__host__ __device__ inline uint32_t comp_step_key(uint32_t k, int s) {
+  return mix32(k * 2u + uint32_t(s));
+}
+// the code of compensation comp of value c with rounding hash h (h = 0 rounds down: exact for a
+// compensation already on the grid, as in uploads)
+__host__ __device__ inline CompCode comp_encode(float comp, float c, uint32_t h) {
+  const float x = ldexpf(comp, kCompShift - comp_exp(c));
+  float v = floorf(x + float(h >> 16) * 0x1p-16f);
+  v = fminf(fmaxf(v, -128.f), 127.f);
+  return CompCode(int(v));
 }
 // effective stored concentration of cell i of a buffer of n cells (mixed: value - compensation)
 template <class F> __device__ inline double conc_effective(const typename F::C *buf, long i, long n);
@@ -63,11 +86,11 @@ template <> __device__ inline double conc_effective<FormF64>(const double *buf, 
   return buf[i];
 }
 // compensation codes of a mixed concentration buffer of n cells
-__host__ __device__ inline const int16_t *comp_codes(const float *buf, long n) {
-  return reinterpret_cast<const int16_t *>(buf + n);
+__host__ __device__ inline const CompCode *comp_codes(const float *buf, long n) {
+  return reinterpret_cast<const CompCode *>(buf + n);
 }
-__host__ __device__ inline int16_t *comp_codes(float *buf, long n) {
-  return reinterpret_cast<int16_t *>(buf + n);
+__host__ __device__ inline CompCode *comp_codes(float *buf, long n) {
+  return reinterpret_cast<CompCode *>(buf + n);
 }
 template <> __device__ inline double conc_effective<FormMixed>(const float *buf, long i, long n) {
   return double(buf[i]) - double(comp_decode(comp_codes(buf, n)[i], buf[i]));
@@ -103,7 +126,8 @@ template <class A> struct PFConsts {
 template <class A> struct DiffConsts {
   A pk[3][3], pc[3][3];
   A D[3];
-  A r;  // dt_d / (dx*dx)
+  A r;           // dt_d / (dx*dx)
+  uint32_t key;  // mixed form: the sub-step's compensation rounding key (comp_step_key)
 };
 
 // Slot-row storage of the ids: KI 16-bit codes per cell.

@@ -15,10 +15,13 @@
 //                          flight per lane, bulk cells skip the partition and weights
 //   diffuse_linear_kernel  the same arithmetic, one thread per cell in storage order (small domains)
 //   diffuse_bulk_vec_kernel    bulk cells only, 16-byte runs per lane, U planes per batch
-//   diffuse_bulk_march_kernel  bulk cells only, 16-byte runs per lane, plane-marching with the
+//   diffuse_bulk_march_kernel  the bulk update of the most common bulk phase for every cell, no
+//                          per-cell flag read, 16-byte runs per lane, plane-marching with the
 //                          tile's rows in LDS (the default sub-step when nx is a multiple of the
 //                          run length; its tile height and z-chunk are picked by timing)
-//   diffuse_general_kernel the full formula over the per-sweep list of non-bulk cells
+//   diffuse_general_kernel the full formula over the per-sweep list of the cells the bulk kernels
+//                          leave (non-bulk cells; with the plane-marching kernel, every cell
+//                          whose flag is not its bulk phase)
 //   diffuse_kernel         full formula for every cell, LDS-tiled (selected with
 //                          PFAMD_DIFF_KERNEL=lds; kept as an independent path for the parity tests)
 //
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
         for (int p = 0; p < P; ++p) wmz[p] = wpz[p];
       }
       del *= dc.r;
-      conc_update<F>(cin, cout, col + plane * z, plane * nz, ccur, del);
+      conc_update<F>(cin, cout, col + plane * z, plane * nz, ccur, del, dc.key);
     }
     prv = cur;
     cur = nxt;
@@ -196,12 +199,16 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
 // Two passes: unit_phase_kernel streams value_phase once and writes one byte per cell (p when the
 // cell's value_phase is exactly e_p and bit p of `mask` is set, else 0); classify_kernel then
 // compares each cell's byte with its stencil's.  The bytes (1/16 of FP64 value_phase) are what the
-// stencil re-reads, from cache.
+// stencil re-reads, from cache.  classify_kernel also counts the bulk cells of each phase into
+// count[1..3] (zeroed by unit_phase_kernel, which runs first on the same stream): the phase with
+// the most bulk cells is the one the plane-marching kernel applies to every cell (bulk_phase).
 template <class T, int P>
 __global__ __launch_bounds__(256) void unit_phase_kernel(const T *__restrict__ pfrac,
                                                          const long ncell, const unsigned mask,
-                                                         uint8_t *__restrict__ unit) {
+                                                         uint8_t *__restrict__ unit,
+                                                         unsigned *__restrict__ count) {
   const long cell = long(blockIdx.x) * 256 + threadIdx.x;
+  if (cell < 4) count[cell] = 0u;
   if (cell >= ncell) return;
   int p = 0, ones = 0, zeros = 0;
 #pragma unroll
@@ -218,48 +225,59 @@ __global__ __launch_bounds__(256) void unit_phase_kernel(const T *__restrict__ p
   unit[cell] = uint8_t(p);
 }
 
+constexpr int kClassifyBlocks = 4096;  // grid-stride, so the per-phase counts take few atomics
+
 template <int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict__ unit,
                                                        uint8_t *__restrict__ flag, const int nx,
                                                        const int ny, const int nz,
                                                        const long ncell,
-                                                       int16_t *__restrict__ codes0,
-                                                       int16_t *__restrict__ codes1) {
-  const long cell = long(blockIdx.x) * 256 + threadIdx.x;
-  if (cell >= ncell) return;
-  int x, y, z;
-  cell_coords(cell, nx, ny, x, y, z);
-  int p = unit[cell];
-  if (p != 0) {
-    const long plane = long(nx) * ny;
-    long nb[6];
-    nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
-    nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;
-    nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
-    nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
-    const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
-    const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
-    nb[4] = x + long(nx) * y + plane * zm;
-    nb[5] = x + long(nx) * y + plane * zp;
+                                                       unsigned *__restrict__ count) {
+  unsigned n1 = 0, n2 = 0, n3 = 0;  // wave-uniform
+  const long stride = long(gridDim.x) * 256;
+  for (long base = long(blockIdx.x) * 256; base < ncell; base += stride) {
+    const long cell = base + threadIdx.x;
+    int p = 0;
+    if (cell < ncell) {
+      int x, y, z;
+      cell_coords(cell, nx, ny, x, y, z);
+      p = unit[cell];
+      if (p != 0) {
+        const long plane = long(nx) * ny;
+        long nb[6];
+        nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
+        nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;
+        nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
+        nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
+        const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+        const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+        nb[4] = x + long(nx) * y + plane * zm;
+        nb[5] = x + long(nx) * y + plane * zp;
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
-      if (unit[nb[f]] != p) p = 0;
+        for (int f = 0; f < NF; ++f)
+          if (unit[nb[f]] != p) p = 0;
+      }
+      flag[cell] = uint8_t(p);
+    }
+    n1 += unsigned(__popcll(__ballot(p == 1)));
+    n2 += unsigned(__popcll(__ballot(p == 2)));
+    n3 += unsigned(__popcll(__ballot(p == 3)));
   }
-  flag[cell] = uint8_t(p);
-  // mixed form: the flag also lives in the low bits of each solute's compensation code
-  if (codes0) codes0[cell] = int16_t((codes0[cell] & ~3) | p);
-  if (codes1) codes1[cell] = int16_t((codes1[cell] & ~3) | p);
+  if ((threadIdx.x & 63) == 0) {
+    if (n1) atomicAdd(count + 1, n1);
+    if (n2) atomicAdd(count + 2, n2);
+    if (n3) atomicAdd(count + 3, n3);
+  }
 }
 
 template <class T, int P, int NF, bool NOFLUX>
 hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
-                           int ny, int nz, long ncell, unsigned mask, int16_t *codes0,
-                           int16_t *codes1) {
+                           int ny, int nz, long ncell, unsigned mask, unsigned *count) {
   const unsigned nb = unsigned((ncell + 255) / 256);
   hipLaunchKernelGGL((unit_phase_kernel<T, P>), dim3(nb), dim3(256), 0, st, pfrac, ncell, mask,
-                     unit);
-  hipLaunchKernelGGL((classify_kernel<NF, NOFLUX>), dim3(nb), dim3(256), 0, st, unit, flag, nx, ny,
-                     nz, ncell, codes0, codes1);
+                     unit, count);
+  hipLaunchKernelGGL((classify_kernel<NF, NOFLUX>), dim3(std::min<unsigned>(nb, kClassifyBlocks)),
+                     dim3(256), 0, st, unit, flag, nx, ny, nz, ncell, count);
   return hipGetLastError();
 }
 
@@ -346,7 +364,7 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
             del += (p - c) * Dp;
           }
           del *= dc.r;
-          conc_update<F>(cin, cout, plane * (z + u) + oc, plane * nz, cc[u], del);
+          conc_update<F>(cin, cout, plane * (z + u) + oc, plane * nz, cc[u], del, dc.key);
         } else {
           general |= 1u << u;
         }
@@ -379,11 +397,13 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
 template <class F, int V>
 __device__ inline void vec_update(const typename F::C *__restrict__ in, typename F::C *__restrict__ out,
                                   long i, long n, const VecA<typename F::A, V> &c,
-                                  const typename F::A (&del)[V], const VecCode<V> &codes);
+                                  const typename F::A (&del)[V], const VecCode<V> &codes,
+                                  uint32_t key);
 template <>
 __device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double *__restrict__ out,
                                               long i, long, const VecA<double, 2> &c,
-                                              const double (&del)[2], const VecCode<2> &) {
+                                              const double (&del)[2], const VecCode<2> &,
+                                              uint32_t) {
   VecA<double, 2> o;
   o.v[0] = c.v[0] + del[0];
   o.v[1] = c.v[1] + del[1];
@@ -392,7 +412,8 @@ __device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double
 template <>
 __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, float *__restrict__ out,
                                                 long i, long n, const VecA<float, 4> &c,
-                                                const float (&del)[4], const VecCode<4> &qi) {
+                                                const float (&del)[4], const VecCode<4> &qi,
+                                                uint32_t key) {
   VecA<float, 4> o;
   VecCode<4> qo;
 #pragma unroll
@@ -401,7 +422,7 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
     const float y = del[v] - comp;
     const float t = c.v[v] + y;
     o.v[v] = t;
-    qo.q[v] = comp_encode((t - c.v[v]) - y, t, qi.q[v]);
+    qo.q[v] = comp_encode((t - c.v[v]) - y, t, mix32(uint32_t(i + v) ^ key));
   }
   *reinterpret_cast<VecA<float, 4> *>(out + i) = o;
   *reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i) = qo;
@@ -443,19 +464,14 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
     VA cc[U], ym[U], yp[U];
     A xm[U], xp[U];
     VecFlag<V> fl[U];
-    VecCode<V> qc[U];  // mixed: compensation codes, whose low bits carry the flags
+    VecCode<V> qc[U];  // mixed: compensation codes
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long base = plane * min(z + u, z1 - 1);
       const typename F::C *__restrict__ pl = cin + base;
-      if constexpr (F::kMixed) {
-        qc[u] = *reinterpret_cast<const VecCode<V> *>(
-            reinterpret_cast<const int16_t *>(cin + ncell) + base + oc);
-#pragma unroll
-        for (int v = 0; v < V; ++v) fl[u].f[v] = uint8_t(qc[u].q[v] & 3);
-      } else {
-        fl[u] = *reinterpret_cast<const VecFlag<V> *>(flag + base + oc);
-      }
+      if constexpr (F::kMixed)
+        qc[u] = *reinterpret_cast<const VecCode<V> *>(comp_codes(cin, ncell) + base + oc);
+      fl[u] = *reinterpret_cast<const VecFlag<V> *>(flag + base + oc);
       cc[u] = ldv(pl, oc);
       xm[u] = A(pl[oxm]);
       xp[u] = A(pl[oxp]);
@@ -498,7 +514,7 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
           }
           del[v] = d * dc.r;
         }
-        vec_update<F, V>(cin, cout, plane * (z + u) + oc, ncell, cc[u], del, qc[u]);
+        vec_update<F, V>(cin, cout, plane * (z + u) + oc, ncell, cc[u], del, qc[u], dc.key);
       }
     }
   }
@@ -523,26 +539,25 @@ __device__ inline void lds_barrier() {
 // traffic on (2/BY of the rows here).  Loads are issued in the order they are consumed: this
 // plane's edge rows and x ends first, then plane z+2 and its tags, so the wait before the update
 // covers only loads from the previous iteration and the prefetch stays in flight across the
-// barrier.  Same per-cell arithmetic and face order as diffuse_bulk_vec_kernel (-x, +x, -y, +y,
-// -z, +z), so the two are bitwise interchangeable; cells with flag 0 get a value
-// diffuse_general_kernel overwrites.  Needs nx % V == 0; partial tiles (x or y) are masked but
-// keep every barrier.
+// barrier.  No per-cell flag is read: every cell gets the bulk update of the phase with the most
+// bulk cells (bulk_phase), and the cells whose flag is another value are listed and overwritten by
+// diffuse_general_kernel.  A cell moves only its value (and, mixed, its one-byte compensation
+// code): FP64 16 B, mixed 10 B per sub-step.  Same per-cell arithmetic and face order as
+// diffuse_bulk_vec_kernel (-x, +x, -y, +y, -z, +z), so the two are bitwise interchangeable on the
+// cells both keep.  Needs nx % V == 0; partial tiles (x or y) are masked but keep every barrier.
 template <class F, int P, int NF, bool NOFLUX, int BY, int V, int R>
 __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
-    const uint8_t *__restrict__ flag, const DiffConsts<typename F::A> dc, const int nx,
+    const unsigned *__restrict__ count, const DiffConsts<typename F::A> dc, const int nx,
     const int ny, const int nz, const int zchunk, const int tiles_x, const int tiles_y,
     const int nblocks) {
   using A = typename F::A;
   using VA = VecA<A, V>;
   static_assert(BY % R == 0, "rows per wavefront must divide the tile height");
-  // per-cell tags of a plane as raw words: the compensation codes (mixed, V 16-bit codes with the
-  // flag in their low bits) or the V flag bytes.  Kept packed until the plane is updated (see the
-  // note in the plane step).
-  constexpr int kTagWords = F::kMixed ? (2 * V + 3) / 4 : 1;
-  using TagWord = std::conditional_t<F::kMixed, uint32_t, uint16_t>;
+  // per-cell tags of a plane: the mixed form's V one-byte compensation codes as one raw word, kept
+  // packed until the plane is updated (see the note in the plane step); none for FP64
   struct Tag {
-    TagWord w[kTagWords];
+    uint32_t w;
   };
   static_assert(F::kMixed ? V == 4 : V == 2, "tag words assume 16-byte runs");
   // the R rows of one lane in one plane
@@ -568,6 +583,12 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
   const long plane = long(nx) * ny;
   const long ncell = plane * nz;
   const int xc = x0 < nx ? x0 : tx0;
+  // the diffusivity of the bulk phase applied to every cell (wave-uniform)
+  const int pd = bulk_phase(count);
+  A Dp = dc.D[0];
+#pragma unroll
+  for (int q = 1; q < P; ++q)
+    if (pd == q + 1) Dp = dc.D[q];
   // per row r of the lane: tile row tr = wv*R + r, domain row y (rows past the domain load a
   // valid row and store nothing)
   int oc[R], oym[R], oyp[R], oxm[R], oxp[R];
@@ -598,16 +619,10 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
     for (int r = 0; r < R; ++r) d.c[r] = ldv(z, oc[r]);
   };
   auto ldtags = [&](int z, Tags &d) {
+    if constexpr (F::kMixed) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if constexpr (F::kMixed) {
-        const uint2 u = *reinterpret_cast<const uint2 *>(
-            reinterpret_cast<const int16_t *>(cin + ncell) + plane * z + oc[r]);
-        d.t[r].w[0] = u.x;
-        d.t[r].w[1] = u.y;
-      } else {
-        d.t[r].w[0] = *reinterpret_cast<const uint16_t *>(flag + plane * z + oc[r]);
-      }
+      for (int r = 0; r < R; ++r)
+        d.t[r].w = *reinterpret_cast<const uint32_t *>(comp_codes(cin, ncell) + plane * z + oc[r]);
     }
   };
   // Register rings, indexed by the plane's position in the chunk modulo 4: planes z-1, z, z+1 are
@@ -655,31 +670,19 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
       // consume the x loads here, outside the store's branch, so they are not sunk behind the
       // prefetch (a wait for a load issued after the prefetch would wait for the prefetch too)
       asm volatile("" ::"v"(xm[r]), "v"(xp[r]));
-      // The empty asm pins the tag words as loaded: without it the compiler unpacks the ring's
+      // The empty asm pins the tag word as loaded: without it the compiler unpacks the ring's
       // tags at the loop's back edge, which waits there for the tags still being prefetched.
-      Tag tg = tc.t[r];
-#pragma unroll
-      for (int k = 0; k < kTagWords; ++k) asm volatile("" : "+v"(tg.w[k]));
       VecCode<V> qc{};
-      int fl[V];
+      if constexpr (F::kMixed) {
+        Tag tg = tc.t[r];
+        asm volatile("" : "+v"(tg.w));
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        if constexpr (F::kMixed) {
-          qc.q[v] = int16_t(tg.w[v >> 1] >> (16 * (v & 1)));
-          fl[v] = qc.q[v] & 3;
-        } else {
-          fl[v] = (int(tg.w[0]) >> (8 * v)) & 0xff;
-        }
+        for (int v = 0; v < V; ++v) qc.q[v] = CompCode(tg.w >> (8 * v));
       }
       if (act[r]) {
         A del[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          const int fv = fl[v];
-          A Dp = dc.D[0];
-#pragma unroll
-          for (int q = 1; q < P; ++q)
-            if (fv == q + 1) Dp = dc.D[q];
           const A c = pc.c[r].v[v];
           const A l = v == 0 ? xm[r] : pc.c[r].v[v > 0 ? v - 1 : 0];
           const A rr = v == V - 1 ? xp[r] : pc.c[r].v[v + 1 < V ? v + 1 : v];
@@ -696,7 +699,7 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
           }
           del[v] = d * dc.r;
         }
-        vec_update<F, V>(cin, cout, plane * z + oc[r], ncell, pc.c[r], del, qc);
+        vec_update<F, V>(cin, cout, plane * z + oc[r], ncell, pc.c[r], del, qc, dc.key);
       }
     }
     // plane z+1 into the other buffer (unread after the chunk's last plane); its readers wait at
@@ -754,7 +757,7 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
 #pragma unroll
     for (int f = 0; f < NF; ++f) del += (nv[f] - c) * Dp;
     del *= dc.r;
-    conc_update<F>(cin, cout, i, plane * nz, cv, del);
+    conc_update<F>(cin, cout, i, plane * nz, cv, del, dc.key);
   } else if (!SPLIT) {
     full_update<F, P, NF>(cin, cout, pfrac, i, plane * nz, nbi, dc);
   }
@@ -787,18 +790,23 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
   }
 }
 
-// ---- list of the cells that need the full formula (flag 0), in storage order -------------------
+// ---- list of the cells that need the full formula, in storage order -----------------------------
+__device__ inline bool listed_cell(int f, const unsigned *count, int keep) {
+  return count ? f != keep : f == 0;
+}
 // Blocks of kListChunk*256 cells.  count: cells per block; scan: exclusive offsets (one
 // workgroup); scatter: storage-order ranks from wave ballots, chunk by chunk.
 __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__restrict__ flag,
+                                                            const unsigned *__restrict__ count,
                                                             long ncell, int *__restrict__ bcount) {
   __shared__ int wsum[4];
   const long base = long(blockIdx.x) * (256L * kListChunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int keep = count ? bulk_phase(count) : 0;
   int n = 0;
   for (int k = 0; k < kListChunk; ++k) {
     const long i = base + long(k) * 256 + threadIdx.x;
-    n += __popcll(__ballot(i < ncell && flag[i] == 0));
+    n += __popcll(__ballot(i < ncell && listed_cell(flag[i], count, keep)));
   }
   if (lane == 0) wsum[wave] = n;
   __syncthreads();
@@ -831,16 +839,18 @@ __global__ __launch_bounds__(1024) void general_scan_kernel(int *__restrict__ bc
 }
 
 __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__restrict__ flag,
+                                                              const unsigned *__restrict__ count,
                                                               long ncell,
                                                               const int *__restrict__ boff,
                                                               int *__restrict__ list) {
   __shared__ int wsum[2][4];
   const long base = long(blockIdx.x) * (256L * kListChunk);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int keep = count ? bulk_phase(count) : 0;
   int run = boff[blockIdx.x];
   for (int k = 0; k < kListChunk; ++k) {
     const long i = base + long(k) * 256 + threadIdx.x;
-    const bool g = i < ncell && flag[i] == 0;
+    const bool g = i < ncell && listed_cell(flag[i], count, keep);
     const unsigned long long m = __ballot(g);
     if (lane == 0) wsum[k & 1][wave] = __popcll(m);
     __syncthreads();  // wsum[k & 1] is rewritten two chunks later, after the next barrier
@@ -854,14 +864,11 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
 // explicit instantiations used by the host dispatch (pf_api.hip)
 #define PF_INST_CLASSIFY(T, P)                                                                 \
   template hipError_t launch_classify<T, P, 4, false>(hipStream_t, const T *, uint8_t *, uint8_t *, \
-                                                      int, int, int, long, unsigned, int16_t *,    \
-                                                      int16_t *);                                  \
+                                                      int, int, int, long, unsigned, unsigned *);  \
   template hipError_t launch_classify<T, P, 6, false>(hipStream_t, const T *, uint8_t *, uint8_t *, \
-                                                      int, int, int, long, unsigned, int16_t *,    \
-                                                      int16_t *);                                  \
+                                                      int, int, int, long, unsigned, unsigned *);  \
   template hipError_t launch_classify<T, P, 6, true>(hipStream_t, const T *, uint8_t *, uint8_t *,  \
-                                                     int, int, int, long, unsigned, int16_t *,     \
-                                                     int16_t *);
+                                                     int, int, int, long, unsigned, unsigned *);
 PF_INST_CLASSIFY(double, 2)
 PF_INST_CLASSIFY(double, 3)
 PF_INST_CLASSIFY(float, 2)
@@ -899,8 +906,8 @@ PF_INST_CLASSIFY(float, 3)
       int);
 #define PF_INST_MARCH_ONE(F, P, NF, NOF, BY, V, R)                                                 \
   template __global__ void diffuse_bulk_march_kernel<F, P, NF, NOF, BY, V, R>(                     \
-      const F::C *, F::C *, const uint8_t *, const DiffConsts<F::A>, int, int, int, int, int, int, \
-      int);
+      const F::C *, F::C *, const unsigned *, const DiffConsts<F::A>, int, int, int, int, int,    \
+      int, int);
 #define PF_INST_MARCH_NF(F, P, BY, V, R)          \
   PF_INST_MARCH_ONE(F, P, 4, false, BY, V, R)     \
   PF_INST_MARCH_ONE(F, P, 6, false, BY, V, R)     \

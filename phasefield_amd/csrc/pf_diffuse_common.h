@@ -48,25 +48,25 @@ template <class A> __device__ inline A face_weight(A prod) {
 
 // total_conc += del for cell i (solver.cpp:366) from its stored value c read from `in` (n cells
 // per buffer).  Mixed form: Kahan-compensated, y = del - comp; t = c + y; comp = (t - c) - y;
-// c = t, with the compensation code stored after the n values of the buffer.
+// c = t, with the compensation code stored after the n values of the buffer and rounded with the
+// sub-step's key.
 template <class F>
 __device__ inline void conc_update(const typename F::C *__restrict__ in,
                                    typename F::C *__restrict__ out, long i, long n,
-                                   typename F::C c, typename F::A del);
+                                   typename F::C c, typename F::A del, uint32_t key);
 template <>
 __device__ inline void conc_update<FormF64>(const double *__restrict__, double *__restrict__ out,
-                                            long i, long, double c, double del) {
+                                            long i, long, double c, double del, uint32_t) {
   out[i] = c + del;
 }
 template <>
 __device__ inline void conc_update<FormMixed>(const float *__restrict__ in, float *__restrict__ out,
-                                              long i, long n, float c, float del) {
-  const int16_t code = comp_codes(in, n)[i];
-  const float comp = comp_decode(code, c);
+                                              long i, long n, float c, float del, uint32_t key) {
+  const float comp = comp_decode(comp_codes(in, n)[i], c);
   const float y = del - comp;
   const float t = c + y;
   out[i] = t;
-  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, code);
+  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, mix32(uint32_t(i) ^ key));
 }
 
 // The reference's sub-step for one cell from its own and its stencil's stored concentration and
@@ -97,7 +97,7 @@ __device__ inline void full_update(const typename F::C *__restrict__ cin,
       del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
   }
   del *= dc.r;
-  conc_update<F>(cin, cout, i, n, r.c, del);
+  conc_update<F>(cin, cout, i, n, r.c, del, dc.key);
 }
 
 template <class A, int V> struct alignas(sizeof(A) * V) VecA {
@@ -106,8 +106,8 @@ template <class A, int V> struct alignas(sizeof(A) * V) VecA {
 template <int V> struct alignas(V) VecFlag {
   uint8_t f[V];
 };
-template <int V> struct alignas(2 * V) VecCode {
-  int16_t q[V];
+template <int V> struct alignas(V) VecCode {
+  CompCode q[V];
 };
 
 }  // namespace pfamd

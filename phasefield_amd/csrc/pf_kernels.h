@@ -57,11 +57,27 @@ __global__ void diffuse_bulk_vec_kernel(const typename F::C *__restrict__ cin,
                                         const int ny, const int nz, const int zchunk,
                                         const int tiles_x, const int tiles_y, const int nblocks);
 
-// plane-marching bulk kernel: same contract as diffuse_bulk_vec_kernel
+// Bulk cell counts per phase of the last classification (count[1..3]; count[0] unused): the phase
+// with the most bulk cells, the first of equal counts, 1 when there are none.  The plane-marching
+// kernel applies that phase's bulk update to every cell; every other cell is listed.
+__host__ __device__ inline int bulk_phase(const unsigned *count) {
+  int p = 1;
+  unsigned best = count[1];
+  if (count[2] > best) {
+    p = 2;
+    best = count[2];
+  }
+  if (count[3] > best) p = 3;
+  return p;
+}
+
+// plane-marching kernel: the bulk update of phase bulk_phase(count) applied to every cell, without
+// reading any per-cell flag; the cells it does not hold for (flag != that phase) are overwritten by
+// diffuse_general_kernel afterwards.  Needs nx % V == 0.
 template <class F, int P, int NF, bool NOFLUX, int BY, int V, int R>
 __global__ void diffuse_bulk_march_kernel(const typename F::C *__restrict__ cin,
                                           typename F::C *__restrict__ cout,
-                                          const uint8_t *__restrict__ flag,
+                                          const unsigned *__restrict__ count,
                                           const DiffConsts<typename F::A> dc, const int nx,
                                           const int ny, const int nz, const int zchunk,
                                           const int tiles_x, const int tiles_y, const int nblocks);
@@ -74,20 +90,22 @@ __global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
                                        const DiffConsts<typename F::A> dc, const int nx,
                                        const int ny, const int nz);
 
+// List of the cells a full-formula pass must cover, in storage order: flag == 0, or, with a
+// non-null `count`, flag != bulk_phase(count) (the cells the plane-marching kernel gets wrong).
 constexpr int kListChunk = 16;  // 256-cell chunks per block of the full-formula list build
-__global__ void general_count_kernel(const uint8_t *__restrict__ flag, long ncell,
+__global__ void general_count_kernel(const uint8_t *__restrict__ flag,
+                                     const unsigned *__restrict__ count, long ncell,
                                      int *__restrict__ bcount);
 __global__ void general_scan_kernel(int *__restrict__ bcount, int nb);
-__global__ void general_scatter_kernel(const uint8_t *__restrict__ flag, long ncell,
+__global__ void general_scatter_kernel(const uint8_t *__restrict__ flag,
+                                       const unsigned *__restrict__ count, long ncell,
                                        const int *__restrict__ boff, int *__restrict__ list);
 
-// Bulk flags of a value_phase field into `flag` (two kernels; `unit` is an N-byte scratch array).
-// codes0/codes1: mixed form, the compensation codes of the solutes' current buffers (their low
-// two bits receive the flag); null for FP64
+// Bulk flags of a value_phase field into `flag` and the bulk cells per phase into count[1..3]
+// (two kernels; `unit` is an N-byte scratch array)
 template <class T, int P, int NF, bool NOFLUX>
 hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
-                           int ny, int nz, long ncell, unsigned mask, int16_t *codes0,
-                           int16_t *codes1);
+                           int ny, int nz, long ncell, unsigned mask, unsigned *count);
 
 template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
 __global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,

@@ -336,6 +336,30 @@ def test_noflux_equals_mirrored_periodic(params3d, rng):
         assert np.max(np.abs(sa["conc"] - sb["conc"][:half])) <= tol
 
 
+def test_compensation_rounding_is_unbiased_and_exact():
+    """The mixed form's one-byte Kahan compensation (multiples of 2^-7 ulp of the value, saturating
+    at -128..127) is rounded stochastically with thresholds from hashed keys: averaged over keys it
+    equals the unrounded compensation (rounding to nearest would return 0 for 0.3 quanta every
+    time), a compensation already on the grid is kept by every key, key 0 rounds down, and the
+    range saturates."""
+    L = oracle.lib()
+    q = lambda comp, value, key: L.orc_quantize_comp(comp, value, key)
+    keys = [L.orc_mix32(i) for i in range(1 << 15)]
+    for value in (0.5, 0.75, 0.02, 1.7, 3.0e-3):
+        e = int(np.frexp(np.float32(value))[1])
+        quantum = float(np.ldexp(1.0, e - 31))
+        for frac in (0.3, -0.3, 0.71, -1.5, 12.125):
+            comp = float(np.float32(frac * quantum))
+            mean = np.mean([q(comp, value, k) for k in keys]) / quantum
+            assert abs(mean - comp / quantum) < 5e-3, (value, frac, mean)
+            assert q(comp, value, 0) == np.floor(comp / quantum) * quantum
+        for n in (-128, -77, -1, 0, 1, 64, 127):
+            comp = n * quantum
+            assert all(q(comp, value, k) == comp for k in keys[:512])
+        assert q(200 * quantum, value, keys[7]) == 127 * quantum
+        assert q(-200 * quantum, value, keys[7]) == -128 * quantum
+
+
 def test_mixed_form_close_to_fp64(setup_state, params3d):
     """The mixed form (FP32 + Kahan, FP64 decisions) tracks the FP64 form closely."""
     (ids, val, phase, conc), d = setup_state
