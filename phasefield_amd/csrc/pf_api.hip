@@ -68,6 +68,7 @@ struct pf_handle {
   bool flag_valid = false;   // flags describe the current value_phase and stencil
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
+  bool tune_log = false;     // print the plane-marching shape timings (PFAMD_TUNE_LOG=1)
   bool diff_split = true;    // full-formula cells from a per-sweep list in their own kernel
   bool diff_vec = true;      // bulk-only sub-step with 16-byte runs per lane (when nx allows)
   bool diff_march = true;    // ... in its plane-marching form
@@ -284,14 +285,13 @@ DiffGeom diff_geom(const pf_handle *h, int rows, long target, int min_chunk, int
 }
 
 // The plane-marching bulk kernel.  Its speed depends on how many workgroups are resident per CU
-// and how long each one marches (at 512^3 the mixed form's sub-step ranges over 357-432 us across
-// tile heights and z-chunks, and the best shape differs between the forms and sizes), which no
+// and how long each one marches (the best shape differs between the forms and sizes), which no
 // closed rule captured.  So at the first sub-step of a handle each candidate tile height (8, 16
-// rows, one per wavefront) and z-chunk (8..128 planes) is launched and timed into the output
-// buffer, and the fastest is kept.  Every candidate writes the same bits (bulk cells; the flag-0
-// cells are rewritten by the listed pass), so the choice changes speed only.  PFAMD_DIFF_BY (8, 16,
-// or 32: two rows per wavefront, which halves the re-read edge rows but measured 6-7 % slower at
-// 512^3 for its 115 VGPRs) and PFAMD_DIFF_ZCHUNK pin either choice.
+// rows, one per wavefront) and z-chunk (8..256 planes) is launched and timed into the output
+// buffer.  Every candidate writes the same bits (the listed pass rewrites the cells the kernel
+// gets wrong), so the choice changes speed only.  PFAMD_DIFF_BY (8, 16, or 32: two rows per
+// wavefront, which halves the re-read edge rows but measured 6-7 % slower at 512^3 for its
+// VGPRs) and PFAMD_DIFF_ZCHUNK pin either choice; PFAMD_TUNE_LOG=1 prints the timings.
 template <class F, int P, int NF, bool NOF>
 hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
                         const typename F::C *cin, typename F::C *cout) {
@@ -322,7 +322,7 @@ hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
     if (h->diff_zchunk_override > 0) {
       zcs = {std::min(h->diff_zchunk_override, nz)};
     } else {
-      for (int zc : {8, 16, 32, 64, 128}) {
+      for (int zc : {8, 16, 32, 64, 128, 256}) {
         const int c = std::min(zc, nz);
         if (std::find(zcs.begin(), zcs.end(), c) == zcs.end()) zcs.push_back(c);
       }
@@ -338,23 +338,44 @@ hipError_t launch_march(pf_handle *h, const DiffConsts<typename F::A> &dc,
         (void)hipEventDestroy(e0);
         return err;
       }
-      float best = 0.f;
+      // each shape: one warm-up launch, then the best of three timed launches; the kept shape is
+      // the longest z-chunk (then the taller tile) within 3 % of the fastest, since longer chunks
+      // re-read fewer end planes and the timings of close shapes differ by noise
+      struct Cand {
+        int by, zc;
+        float ms;
+      };
+      std::vector<Cand> cands;
       for (int by : bys)
         for (int zc : zcs) {
-          float ms = 0.f;
           if ((err = launch(by, zc)) != hipSuccess) break;  // warm-up
-          if ((err = hipEventRecord(e0, h->stream)) != hipSuccess) break;
-          for (int r = 0; r < 2 && err == hipSuccess; ++r) err = launch(by, zc);
-          if (err != hipSuccess) break;
-          if ((err = hipEventRecord(e1, h->stream)) != hipSuccess) break;
-          if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
-          if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-          if (h->march_by == 0 || ms < best) {
-            best = ms;
-            h->march_by = by;
-            h->march_zc = zc;
+          float best_ms = 0.f;
+          for (int r = 0; r < 3 && err == hipSuccess; ++r) {
+            float ms = 0.f;
+            if ((err = hipEventRecord(e0, h->stream)) != hipSuccess) break;
+            if ((err = launch(by, zc)) != hipSuccess) break;
+            if ((err = hipEventRecord(e1, h->stream)) != hipSuccess) break;
+            if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+            if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            if (r == 0 || ms < best_ms) best_ms = ms;
           }
+          if (err != hipSuccess) break;
+          cands.push_back({by, zc, best_ms});
+          if (h->tune_log)
+            std::fprintf(stderr, "[pfamd] march %s rows %d z-chunk %d: %.1f us\n",
+                         F::kMixed ? "mixed" : "fp64", by, zc, 1e3 * best_ms);
         }
+      if (err == hipSuccess && !cands.empty()) {
+        float fastest = cands[0].ms;
+        for (const Cand &c : cands) fastest = std::min(fastest, c.ms);
+        const Cand *pick = nullptr;
+        for (const Cand &c : cands)
+          if (c.ms <= 1.03f * fastest &&
+              (!pick || c.zc > pick->zc || (c.zc == pick->zc && c.by > pick->by)))
+            pick = &c;
+        h->march_by = pick->by;
+        h->march_zc = pick->zc;
+      }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       if (err != hipSuccess) {
@@ -585,6 +606,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if (const char *e = std::getenv("PFAMD_DIFF_KERNEL"))
     h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
   if (const char *e = std::getenv("PFAMD_DIFF_BY")) h->diff_by = std::atoi(e);
+  if (const char *e = std::getenv("PFAMD_TUNE_LOG")) h->tune_log = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_DIFF_SPLIT")) h->diff_split = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_DIFF_VEC")) h->diff_vec = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_DIFF_MARCH")) h->diff_march = std::atoi(e) != 0;
