@@ -772,12 +772,10 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
     const int *__restrict__ count, const DiffConsts<typename F::A> dc, const int nx,
     const int ny, const int nz) {
   const int n = *count;
-  const long plane = long(nx) * ny;
-  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-    const long i = list[k];
+  const long plane = long(nx) * ny, ncell = plane * nz;
+  auto stencil = [&](long i, long (&nbi)[6]) {
     int x, y, z;
     cell_coords(i, nx, ny, x, y, z);
-    long nbi[6];
     nbi[0] = i - x + wrap(x - 1, nx);
     nbi[1] = i - x + wrap(x + 1, nx);
     nbi[2] = i + long(nx) * (wrap(y - 1, ny) - y);
@@ -786,7 +784,21 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
     const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
     nbi[4] = i + plane * (zm - z);
     nbi[5] = i + plane * (zp - z);
-    full_update<F, P, NF>(cin, cout, pfrac, i, plane * nz, nbi, dc);
+  };
+  // two list entries per iteration, the second's loads issued before the first's arithmetic (the
+  // pass is bound by the latency of its gathers, not by bandwidth)
+  const int stride = gridDim.x * 256;
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += 2 * stride) {
+    const bool two = k + stride < n;
+    const long i0 = list[k], i1 = two ? long(list[k + stride]) : i0;
+    long nb0[6], nb1[6];
+    stencil(i0, nb0);
+    stencil(i1, nb1);
+    StencilLoad<F, P, NF> s0, s1;
+    stencil_load<F, P, NF>(cin, pfrac, i0, ncell, nb0, s0);
+    stencil_load<F, P, NF>(cin, pfrac, i1, ncell, nb1, s1);
+    stencil_update<F, P, NF>(cout, i0, ncell, s0, dc);
+    if (two) stencil_update<F, P, NF>(cout, i1, ncell, s1, dc);
   }
 }
 

@@ -46,58 +46,91 @@ template <class A> __device__ inline A face_weight(A prod) {
   return w;
 }
 
-// total_conc += del for cell i (solver.cpp:366) from its stored value c read from `in` (n cells
-// per buffer).  Mixed form: Kahan-compensated, y = del - comp; t = c + y; comp = (t - c) - y;
-// c = t, with the compensation code stored after the n values of the buffer and rounded with the
-// sub-step's key.
+// total_conc += del for cell i (solver.cpp:366) of stored value c, written to `out` (n cells per
+// buffer).  Mixed form: Kahan-compensated with the cell's compensation code q, y = del - comp;
+// t = c + y; comp = (t - c) - y; c = t, the new code stored after the n values of the buffer and
+// rounded with the sub-step's key.
 template <class F>
-__device__ inline void conc_update(const typename F::C *__restrict__ in,
-                                   typename F::C *__restrict__ out, long i, long n,
-                                   typename F::C c, typename F::A del, uint32_t key);
+__device__ inline void conc_store(typename F::C *__restrict__ out, long i, long n, typename F::C c,
+                                  typename F::A del, CompCode q, uint32_t key);
 template <>
-__device__ inline void conc_update<FormF64>(const double *__restrict__, double *__restrict__ out,
-                                            long i, long, double c, double del, uint32_t) {
+__device__ inline void conc_store<FormF64>(double *__restrict__ out, long i, long, double c,
+                                           double del, CompCode, uint32_t) {
   out[i] = c + del;
 }
 template <>
-__device__ inline void conc_update<FormMixed>(const float *__restrict__ in, float *__restrict__ out,
-                                              long i, long n, float c, float del, uint32_t key) {
-  const float comp = comp_decode(comp_codes(in, n)[i], c);
+__device__ inline void conc_store<FormMixed>(float *__restrict__ out, long i, long n, float c,
+                                             float del, CompCode q, uint32_t key) {
+  const float comp = comp_decode(q, c);
   const float y = del - comp;
   const float t = c + y;
   out[i] = t;
   comp_codes(out, n)[i] = comp_encode((t - c) - y, t, mix32(uint32_t(i) ^ key));
 }
+// the same with the code read from the input buffer `in`
+template <class F>
+__device__ inline void conc_update(const typename F::C *__restrict__ in,
+                                   typename F::C *__restrict__ out, long i, long n,
+                                   typename F::C c, typename F::A del, uint32_t key) {
+  CompCode q = 0;
+  if constexpr (F::kMixed) q = comp_codes(in, n)[i];
+  conc_store<F>(out, i, n, c, del, q, key);
+}
 
-// The reference's sub-step for one cell from its own and its stencil's stored concentration and
-// value_phase: partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365), update.
+// The stored values the reference's sub-step of one cell reads: its own and its stencil's
+// concentration and value_phase (and, mixed, its compensation code), loaded before any arithmetic
+// so that a thread can have the loads of several cells in flight.
+template <class F, int P, int NF> struct StencilLoad {
+  RawCell<F, P> me, nb[NF];
+  CompCode q;
+};
 template <class F, int P, int NF>
-__device__ inline void full_update(const typename F::C *__restrict__ cin,
-                                   typename F::C *__restrict__ cout,
-                                   const typename F::T *__restrict__ pfrac, long i, long n,
-                                   const long (&nbi)[6], const DiffConsts<typename F::A> &dc) {
-  using A = typename F::A;
-  RawCell<F, P> r;
-  r.c = cin[i];
+__device__ inline void stencil_load(const typename F::C *__restrict__ cin,
+                                    const typename F::T *__restrict__ pfrac, long i, long n,
+                                    const long (&nbi)[6], StencilLoad<F, P, NF> &s) {
+  s.me.c = cin[i];
 #pragma unroll
-  for (int p = 0; p < P; ++p) r.ph[p] = pfrac[i * P + p];
+  for (int p = 0; p < P; ++p) s.me.ph[p] = pfrac[i * P + p];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    s.nb[f].c = cin[nbi[f]];
+#pragma unroll
+    for (int p = 0; p < P; ++p) s.nb[f].ph[p] = pfrac[nbi[f] * P + p];
+  }
+  s.q = 0;
+  if constexpr (F::kMixed) s.q = comp_codes(cin, n)[i];
+}
+// partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365), update
+template <class F, int P, int NF>
+__device__ inline void stencil_update(typename F::C *__restrict__ cout, long i, long n,
+                                      const StencilLoad<F, P, NF> &s,
+                                      const DiffConsts<typename F::A> &dc) {
+  using A = typename F::A;
   CellPF<A> me;
-  partition_cell<F, P>(r, dc, me);
+  partition_cell<F, P>(s.me, dc, me);
   A del = A(0);
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    RawCell<F, P> rn;
-    rn.c = cin[nbi[f]];
-#pragma unroll
-    for (int p = 0; p < P; ++p) rn.ph[p] = pfrac[nbi[f] * P + p];
     CellPF<A> nb;
-    partition_cell<F, P>(rn, dc, nb);
+    partition_cell<F, P>(s.nb[f], dc, nb);
 #pragma unroll
     for (int p = 0; p < P; ++p)
       del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
   }
   del *= dc.r;
-  conc_update<F>(cin, cout, i, n, r.c, del, dc.key);
+  conc_store<F>(cout, i, n, s.me.c, del, s.q, dc.key);
+}
+
+// The reference's sub-step for one cell from its own and its stencil's stored concentration and
+// value_phase.
+template <class F, int P, int NF>
+__device__ inline void full_update(const typename F::C *__restrict__ cin,
+                                   typename F::C *__restrict__ cout,
+                                   const typename F::T *__restrict__ pfrac, long i, long n,
+                                   const long (&nbi)[6], const DiffConsts<typename F::A> &dc) {
+  StencilLoad<F, P, NF> s;
+  stencil_load<F, P, NF>(cin, pfrac, i, n, nbi, s);
+  stencil_update<F, P, NF>(cout, i, n, s, dc);
 }
 
 template <class A, int V> struct alignas(sizeof(A) * V) VecA {
