@@ -170,6 +170,67 @@ def test_march_noflux_bitwise(tmp_path, monkeypatch, rows, zchunk, form):
     assert_same(gs, os_, 4, f"march noflux {rows}/{zchunk}")
 
 
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("gamma_cols", [48, 16, 0])
+def test_bulk_phase_choice_bitwise(base_params, rng, form, gamma_cols):
+    """The plane-marching kernel applies the bulk update of the phase with the most bulk cells to
+    every cell and the listed pass rewrites the rest.  A single austenite block of gamma_cols of the
+    64 columns and a ferrite block elsewhere: austenite is the bulk phase at 48 columns, ferrite at
+    16, and at 0 every cell is ferrite bulk.  Random concentrations, 37 sub-steps, bitwise."""
+    banner(f"bulk phase choice gamma_cols={gamma_cols} form={form}")
+    nx, ny, nz, K = 64, 24, 16, 4
+    N = nx * ny * nz
+    x = np.tile(np.arange(nx), ny * nz)
+    gamma = x < gamma_cols
+    ids = np.zeros((N, K), np.int32)
+    val = np.zeros((N, K))
+    ids[:, 0] = np.where(gamma, 100001, 200001)
+    val[:, 0] = 1.0
+    phase = np.zeros((N, 3))
+    phase[gamma, 0] = 1.0
+    phase[~gamma, 1] = 1.0
+    conc = 0.02 + 0.8 * rng.random(N)
+    st = (ids.reshape(-1), val.reshape(-1), phase.reshape(-1), conc)
+    gs, os_ = run_pair((nx, ny, nz), K, 2, 1, 0, form, st, base_params,
+                       lambda g, p: g.diffuse(0, 37, p.dt_d[0]),
+                       lambda o, p: o.diffuse(0, 37, p.dt_d[0]))
+    assert_same(gs, os_, K, f"bulk phase {gamma_cols}")
+
+
+def test_mixed_checkpoint_keeps_rounding_keys(tmp_path):
+    """The mixed form rounds its compensation with keys counted in sub-steps since the state was
+    set: a checkpoint carries the count, an upload restarts it.  2 steps + checkpoint + 3 steps +
+    restore + 2 steps equal 4 straight steps, bitwise, compensation codes included."""
+    banner("mixed checkpoint keys")
+    m, p, st = _setup_state(tmp_path, workload="fecmn_3d_512", nx=40, ny=24, nz=20, grains=5,
+                            nuclei=3)
+    pp = oracle.as_orc_params(p)
+    from phasefield_amd._capi import PfParams
+    gp = PfParams.from_buffer_copy(pp)
+    a = Domain(40, 24, 20, depth=4, nphase=2, nsolute=2, precision="mixed")
+    a.set_params(gp)
+    a.upload(*st)
+    a.step(2)
+    blob = a.checkpoint()
+    a.step(3)
+    a.restore(blob)
+    a.step(2)
+    b = Domain(40, 24, 20, depth=4, nphase=2, nsolute=2, precision="mixed")
+    b.set_params(gp)
+    b.upload(*st)
+    b.step(4)
+    ra, rb = a.download(), b.download()
+    assert_same(ra, rb, 4, "restore")
+    assert np.array_equal(ra["conc_comp"], rb["conc_comp"])
+    a.upload(*st)
+    a.step(4)
+    ra = a.download()
+    assert_same(ra, rb, 4, "re-upload")
+    assert np.array_equal(ra["conc_comp"], rb["conc_comp"])
+    a.close()
+    b.close()
+
+
 def test_bulk_reduction_needs_identity_partition(tmp_path):
     """With k(2,2) != 1 the partition of a pure ferrite cell is not its total concentration, so the
     bulk-cell reduction must not be taken for ferrite: results stay bitwise equal to the oracle."""
