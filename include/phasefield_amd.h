@@ -89,7 +89,9 @@ typedef struct pf_host_state {
   double *value;      /* [N][K]  value_grain1 */
   double *phase;      /* [N][3]  value_phase */
   double *conc;       /* [nsolute][N] total concentration (total_conc; solute 1 = Mn) */
-  double *conc_comp;  /* [nsolute][N] Kahan compensation of the mixed form, or NULL */
+  double *conc_comp;  /* [nsolute][N] Kahan compensation of the mixed form (effective value
+                       * conc - conc_comp; stored as a multiple of 2^-7 ulp of conc, an upload
+                       * rounds down to that grid), or NULL */
   double *phase_conc; /* [nsolute][N][3] partitioned concentrations (value_conc1); download only, or NULL */
 } pf_host_state;
 
