@@ -191,9 +191,10 @@ __global__ __launch_bounds__(256) void adj_diffuse_kernel(
 }
 
 // Split form: the listed cells (flag 0 after adj_list_flag_kernel: interface cells and bulk cells
-// of phase 1, whose terms carry dL/dD(phase 1)) in a grid-stride pass, after the bulk cells were
-// updated by the forward bulk kernel run on lambda (for a bulk cell of phase p the reverse
-// sub-step lam + r D_p sum_faces (lam_nb - lam) is that kernel's update with lam for c).
+// of phase 1, whose terms carry dL/dD(phase 1); with the plane-marching kernel also the bulk cells
+// of phases other than its bulk phase) in a grid-stride pass, after the bulk cells were updated by
+// the forward bulk kernel run on lambda (for a bulk cell of phase p the reverse sub-step
+// lam + r D_p sum_faces (lam_nb - lam) is that kernel's update with lam for c).
 template <int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void adj_general_kernel(
     const double *__restrict__ c, const double *__restrict__ pf, const int *__restrict__ list,

@@ -74,7 +74,7 @@ struct pf_handle {
   bool diff_march = true;    // ... in its plane-marching form
   int march_by = 0;          // its tile rows and z-chunk, picked by timing at first use
   int march_zc = 0;
-  int *glist = nullptr;      // indices of the cells with flag 0, storage order
+  int *glist = nullptr;      // indices of the listed cells, storage order
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
   bool sweep_split = true;   // copy cells classified and written apart from the listed sweep
   uint8_t *sflag = nullptr;  // per cell: 1 = copied by the last sweep

@@ -763,8 +763,9 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
   }
 }
 
-// Cells next to a phase boundary (flag 0), from the list built once per sweep: the full
-// partition + weighted flux, one thread per listed cell (grid-stride over the device-side count).
+// The listed cells (next to a phase boundary, and with the plane-marching kernel also the bulk
+// cells of the other phases), from the list built once per sweep: the full partition + weighted
+// flux, grid-stride over the device-side count.
 template <class F, int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void diffuse_general_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
