@@ -394,6 +394,20 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
 // x neighbours at the ends of its run with two scalar loads, and walks a z-chunk U planes at a
 // time.  No branch on the flag: cells with flag 0 get a value that diffuse_general_kernel
 // overwrites afterwards (same stream), so every store is a full vector.  Needs nx % V == 0.
+// Streaming stores of a run (one `global_store ... nt` each): the sub-step's output is not read
+// again in the same launch, and non-temporal stores keep it from displacing the planes the
+// workgroups are still reading (512^3 sub-step: mixed 301 -> 291 us, FP64 445 -> 434 us; the same
+// hint on the loads cost 16-30 %, since it evicts the rows the neighbouring tiles re-read).
+template <class T, int V> __device__ inline void store_nt(VecA<T, V> *p, const VecA<T, V> &v) {
+#pragma unroll
+  for (int k = 0; k < V; ++k) __builtin_nontemporal_store(v.v[k], &p->v[k]);
+}
+template <int V> __device__ inline void store_nt(VecCode<V> *p, const VecCode<V> &q) {
+  static_assert(V == 4, "four one-byte codes per word");
+  __builtin_nontemporal_store(*reinterpret_cast<const uint32_t *>(&q),
+                              reinterpret_cast<uint32_t *>(p));
+}
+
 template <class F, int V>
 __device__ inline void vec_update(const typename F::C *__restrict__ in, typename F::C *__restrict__ out,
                                   long i, long n, const VecA<typename F::A, V> &c,
@@ -407,7 +421,7 @@ __device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double
   VecA<double, 2> o;
   o.v[0] = c.v[0] + del[0];
   o.v[1] = c.v[1] + del[1];
-  *reinterpret_cast<VecA<double, 2> *>(out + i) = o;
+  store_nt(reinterpret_cast<VecA<double, 2> *>(out + i), o);
 }
 template <>
 __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, float *__restrict__ out,
@@ -424,8 +438,8 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
     o.v[v] = t;
     qo.q[v] = comp_encode((t - c.v[v]) - y, t, mix32(uint32_t(i + v) ^ key));
   }
-  *reinterpret_cast<VecA<float, 4> *>(out + i) = o;
-  *reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i) = qo;
+  store_nt(reinterpret_cast<VecA<float, 4> *>(out + i), o);
+  store_nt(reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i), qo);
 }
 
 template <class F, int P, int NF, bool NOFLUX, int BY, int V>
