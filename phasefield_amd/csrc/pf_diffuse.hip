@@ -270,14 +270,78 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict
   }
 }
 
+// The same for nx % 4 == 0, four x-consecutive cells per thread: their own bytes, their -y/+y and
+// -z/+z neighbours are one aligned 32-bit word each, and only the two x ends are single bytes.
+template <int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void classify4_kernel(const uint8_t *__restrict__ unit,
+                                                        uint8_t *__restrict__ flag, const int nx,
+                                                        const int ny, const int nz,
+                                                        const long ncell,
+                                                        unsigned *__restrict__ count) {
+  const uint32_t *__restrict__ u32 = reinterpret_cast<const uint32_t *>(unit);
+  unsigned n1 = 0, n2 = 0, n3 = 0;  // wave-uniform
+  const long nq = ncell / 4, stride = long(gridDim.x) * 256, plane = long(nx) * ny;
+  for (long base = long(blockIdx.x) * 256; base < nq; base += stride) {
+    const long q = base + threadIdx.x;
+    int p[4] = {0, 0, 0, 0};
+    if (q < nq) {
+      const long c = q * 4;
+      int x, y, z;
+      cell_coords(c, nx, ny, x, y, z);
+      const uint32_t me = u32[q];
+      if (me != 0u) {
+        const uint32_t ym = NF >= 4 ? u32[(c + long(nx) * (wrap(y - 1, ny) - y)) / 4] : me;
+        const uint32_t yp = NF >= 4 ? u32[(c + long(nx) * (wrap(y + 1, ny) - y)) / 4] : me;
+        uint32_t zm = me, zp = me;
+        if (NF == 6) {
+          const int zl = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+          const int zh = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+          zm = u32[(c + plane * (zl - z)) / 4];
+          zp = u32[(c + plane * (zh - z)) / 4];
+        }
+        const int xl = unit[c - x + wrap(x - 1, nx)], xr = unit[c - x + wrap(x + 4, nx)];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          int pv = int((me >> (8 * v)) & 0xffu);
+          const int l = v == 0 ? xl : int((me >> (8 * (v - 1))) & 0xffu);
+          const int r = v == 3 ? xr : int((me >> (8 * (v + 1))) & 0xffu);
+          if (l != pv || r != pv) pv = 0;
+          if (int((ym >> (8 * v)) & 0xffu) != pv || int((yp >> (8 * v)) & 0xffu) != pv) pv = 0;
+          if (int((zm >> (8 * v)) & 0xffu) != pv || int((zp >> (8 * v)) & 0xffu) != pv) pv = 0;
+          p[v] = pv;
+        }
+      }
+      reinterpret_cast<uint32_t *>(flag)[q] =
+          uint32_t(p[0]) | uint32_t(p[1]) << 8 | uint32_t(p[2]) << 16 | uint32_t(p[3]) << 24;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      n1 += unsigned(__popcll(__ballot(p[v] == 1)));
+      n2 += unsigned(__popcll(__ballot(p[v] == 2)));
+      n3 += unsigned(__popcll(__ballot(p[v] == 3)));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (n1) atomicAdd(count + 1, n1);
+    if (n2) atomicAdd(count + 2, n2);
+    if (n3) atomicAdd(count + 3, n3);
+  }
+}
+
 template <class T, int P, int NF, bool NOFLUX>
 hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
                            int ny, int nz, long ncell, unsigned mask, unsigned *count) {
   const unsigned nb = unsigned((ncell + 255) / 256);
   hipLaunchKernelGGL((unit_phase_kernel<T, P>), dim3(nb), dim3(256), 0, st, pfrac, ncell, mask,
                      unit, count);
-  hipLaunchKernelGGL((classify_kernel<NF, NOFLUX>), dim3(std::min<unsigned>(nb, kClassifyBlocks)),
-                     dim3(256), 0, st, unit, flag, nx, ny, nz, ncell, count);
+  if (nx % 4 == 0)
+    hipLaunchKernelGGL((classify4_kernel<NF, NOFLUX>),
+                       dim3(std::min<unsigned>((nb + 3) / 4, kClassifyBlocks)), dim3(256), 0, st,
+                       unit, flag, nx, ny, nz, ncell, count);
+  else
+    hipLaunchKernelGGL((classify_kernel<NF, NOFLUX>),
+                       dim3(std::min<unsigned>(nb, kClassifyBlocks)), dim3(256), 0, st, unit, flag,
+                       nx, ny, nz, ncell, count);
   return hipGetLastError();
 }
 
@@ -821,20 +885,38 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
 __device__ inline bool listed_cell(int f, const unsigned *count, int keep) {
   return count ? f != keep : f == 0;
 }
-// Blocks of kListChunk*256 cells.  count: cells per block; scan: exclusive offsets (one
-// workgroup); scatter: storage-order ranks from wave ballots, chunk by chunk.
+// Blocks of 256 threads x kListChunk consecutive cells (one 16-byte load of flags per thread).
+// count: listed cells per block; scan: exclusive offsets (one workgroup); scatter: each thread
+// writes its listed cells in order at the block's offset plus the exclusive sum of the counts of
+// the threads before it, so the list is in storage order.
+static_assert(kListChunk == 16, "one uint4 of flags per thread");
+// listed bits of the thread's kListChunk cells starting at `first` (cells at or past ncell: none)
+__device__ inline unsigned listed_bits(const uint8_t *__restrict__ flag, long first, long ncell,
+                                       const unsigned *count, int keep) {
+  unsigned bits = 0;
+  if (first + kListChunk <= ncell) {
+    const uint4 w = *reinterpret_cast<const uint4 *>(flag + first);
+    const uint32_t word[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int j = 0; j < kListChunk; ++j)
+      if (listed_cell(int((word[j >> 2] >> (8 * (j & 3))) & 0xffu), count, keep)) bits |= 1u << j;
+  } else {
+    for (int j = 0; j < kListChunk && first + j < ncell; ++j)
+      if (listed_cell(flag[first + j], count, keep)) bits |= 1u << j;
+  }
+  return bits;
+}
+
 __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__restrict__ flag,
                                                             const unsigned *__restrict__ count,
                                                             long ncell, int *__restrict__ bcount) {
   __shared__ int wsum[4];
-  const long base = long(blockIdx.x) * (256L * kListChunk);
+  const long first = long(blockIdx.x) * (256L * kListChunk) + long(threadIdx.x) * kListChunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int keep = count ? bulk_phase(count) : 0;
-  int n = 0;
-  for (int k = 0; k < kListChunk; ++k) {
-    const long i = base + long(k) * 256 + threadIdx.x;
-    n += __popcll(__ballot(i < ncell && listed_cell(flag[i], count, keep)));
-  }
+  int n = __popc(listed_bits(flag, first, ncell, count, keep));
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) n += __shfl_xor(n, d);
   if (lane == 0) wsum[wave] = n;
   __syncthreads();
   if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
@@ -870,21 +952,26 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
                                                               long ncell,
                                                               const int *__restrict__ boff,
                                                               int *__restrict__ list) {
-  __shared__ int wsum[2][4];
-  const long base = long(blockIdx.x) * (256L * kListChunk);
+  __shared__ int wsum[4];
+  const long first = long(blockIdx.x) * (256L * kListChunk) + long(threadIdx.x) * kListChunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int keep = count ? bulk_phase(count) : 0;
-  int run = boff[blockIdx.x];
-  for (int k = 0; k < kListChunk; ++k) {
-    const long i = base + long(k) * 256 + threadIdx.x;
-    const bool g = i < ncell && listed_cell(flag[i], count, keep);
-    const unsigned long long m = __ballot(g);
-    if (lane == 0) wsum[k & 1][wave] = __popcll(m);
-    __syncthreads();  // wsum[k & 1] is rewritten two chunks later, after the next barrier
-    int w0 = run;
-    for (int w = 0; w < wave; ++w) w0 += wsum[k & 1][w];
-    if (g) list[w0 + __popcll(m & ((1ull << lane) - 1ull))] = int(i);
-    run += wsum[k & 1][0] + wsum[k & 1][1] + wsum[k & 1][2] + wsum[k & 1][3];
+  unsigned bits = listed_bits(flag, first, ncell, count, keep);
+  const int c = __popc(bits);
+  int v = c;  // inclusive scan over the wave's lanes
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(v, d);
+    if (lane >= d) v += y;
+  }
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  int pos = boff[blockIdx.x] + v - c;
+  for (int w = 0; w < wave; ++w) pos += wsum[w];
+  while (bits) {
+    const int j = __builtin_ctz(bits);
+    bits &= bits - 1;
+    list[pos++] = int(first + j);
   }
 }
 
