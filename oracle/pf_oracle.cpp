@@ -432,7 +432,9 @@ template <class F> struct Domain : DomainBase {
             if (F::kahan) {
               const S y = S(del) - comp[s][c];
               const S t = conc[s][c] + y;
-              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t, mix32(uint32_t(c) ^ key));
+              // cells 2j and 2j + 1 take the high and low halves of mix32(j ^ key)
+              const uint32_t hj = mix32(uint32_t(c >> 1) ^ key);
+              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t, (c & 1) ? hj << 16 : hj);
               conc[s][c] = t;
             } else {
               conc[s][c] += S(del);

@@ -68,9 +68,15 @@ __host__ __device__ inline uint32_t mix32(uint32_t x) {
   return x;
 }
 // key of one sub-step of solute s: k counts the sub-steps applied to that solute since the state
-// was set; the rounding threshold of cell i is then taken from mix32(i ^ key)
+// was set; the rounding thresholds of cells 2j and 2j + 1 are then the high and low halves of
+// mix32(j ^ key) (comp_hash)
 __host__ __device__ inline uint32_t comp_step_key(uint32_t k, int s) {
   return mix32(k * 2u + uint32_t(s));
+}
+// rounding hash of cell i (its threshold in the high 16 bits)
+__host__ __device__ inline uint32_t comp_hash(uint32_t i, uint32_t key) {
+  const uint32_t h = mix32((i >> 1) ^ key);
+  return (i & 1u) ? h << 16 : h;
 }
 // the code of compensation comp of value c with rounding hash h (h = 0 rounds down: exact for a
 // compensation already on the grid, as in uploads)

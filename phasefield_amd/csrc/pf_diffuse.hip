@@ -494,13 +494,16 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
                                                 uint32_t key) {
   VecA<float, 4> o;
   VecCode<4> qo;
+  // i is a multiple of 4: cells i, i+1 share one hash, i+2, i+3 the next (comp_hash)
+  const uint32_t h0 = mix32((uint32_t(i) >> 1) ^ key), h1 = mix32(((uint32_t(i) >> 1) + 1u) ^ key);
+  const uint32_t h[4] = {h0, h0 << 16, h1, h1 << 16};
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const float comp = comp_decode(qi.q[v], c.v[v]);
     const float y = del[v] - comp;
     const float t = c.v[v] + y;
     o.v[v] = t;
-    qo.q[v] = comp_encode((t - c.v[v]) - y, t, mix32(uint32_t(i + v) ^ key));
+    qo.q[v] = comp_encode((t - c.v[v]) - y, t, h[v]);
   }
   store_nt(reinterpret_cast<VecA<float, 4> *>(out + i), o);
   store_nt(reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i), qo);

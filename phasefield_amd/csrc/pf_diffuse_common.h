@@ -65,7 +65,7 @@ __device__ inline void conc_store<FormMixed>(float *__restrict__ out, long i, lo
   const float y = del - comp;
   const float t = c + y;
   out[i] = t;
-  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, mix32(uint32_t(i) ^ key));
+  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, comp_hash(uint32_t(i), key));
 }
 // the same with the code read from the input buffer `in`
 template <class F>
