@@ -225,6 +225,40 @@ __global__ __launch_bounds__(256) void unit_phase_kernel(const T *__restrict__ p
   unit[cell] = uint8_t(p);
 }
 
+// the same for four consecutive cells per thread (ncell % 4 == 0): P*4 values read as vectors, one
+// 32-bit word of unit bytes written
+template <class T, int P>
+__global__ __launch_bounds__(256) void unit_phase4_kernel(const T *__restrict__ pfrac,
+                                                          const long ncell, const unsigned mask,
+                                                          uint8_t *__restrict__ unit,
+                                                          unsigned *__restrict__ count) {
+  const long q = long(blockIdx.x) * 256 + threadIdx.x;
+  if (q < 4) count[q] = 0u;
+  if (q >= ncell / 4) return;
+  struct alignas(16) Quad {
+    T v[4 * P];
+  };
+  const Quad ph = reinterpret_cast<const Quad *>(pfrac)[q];
+  uint32_t out = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    int p = 0, ones = 0, zeros = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const T v = ph.v[c * P + k];
+      if (v == T(1)) {
+        ++ones;
+        p = k + 1;
+      } else if (v == T(0)) {
+        ++zeros;
+      }
+    }
+    if (!(ones == 1 && zeros == P - 1) || !((mask >> p) & 1u)) p = 0;
+    out |= uint32_t(p) << (8 * c);
+  }
+  reinterpret_cast<uint32_t *>(unit)[q] = out;
+}
+
 constexpr int kClassifyBlocks = 4096;  // grid-stride, so the per-phase counts take few atomics
 
 template <int NF, bool NOFLUX>
@@ -332,8 +366,12 @@ template <class T, int P, int NF, bool NOFLUX>
 hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
                            int ny, int nz, long ncell, unsigned mask, unsigned *count) {
   const unsigned nb = unsigned((ncell + 255) / 256);
-  hipLaunchKernelGGL((unit_phase_kernel<T, P>), dim3(nb), dim3(256), 0, st, pfrac, ncell, mask,
-                     unit, count);
+  if (nx % 4 == 0)
+    hipLaunchKernelGGL((unit_phase4_kernel<T, P>), dim3((nb + 3) / 4), dim3(256), 0, st, pfrac,
+                       ncell, mask, unit, count);
+  else
+    hipLaunchKernelGGL((unit_phase_kernel<T, P>), dim3(nb), dim3(256), 0, st, pfrac, ncell, mask,
+                       unit, count);
   if (nx % 4 == 0)
     hipLaunchKernelGGL((classify4_kernel<NF, NOFLUX>),
                        dim3(std::min<unsigned>((nb + 3) / 4, kClassifyBlocks)), dim3(256), 0, st,
