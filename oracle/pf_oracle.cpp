@@ -432,8 +432,8 @@ template <class F> struct Domain : DomainBase {
             if (F::kahan) {
               const S y = S(del) - comp[s][c];
               const S t = conc[s][c] + y;
-              // cells 2j and 2j + 1 take the high and low halves of mix32(j ^ key)
-              const uint32_t hj = mix32(uint32_t(c >> 1) ^ key);
+              // cells 2j and 2j + 1 take the high and low halves of key ^ (j * 0x9E3779B9)
+              const uint32_t hj = key ^ (uint32_t(c >> 1) * 0x9E3779B9u);
               comp[s][c] = quantize_comp((t - conc[s][c]) - y, t, (c & 1) ? hj << 16 : hj);
               conc[s][c] = t;
             } else {

@@ -69,13 +69,15 @@ __host__ __device__ inline uint32_t mix32(uint32_t x) {
 }
 // key of one sub-step of solute s: k counts the sub-steps applied to that solute since the state
 // was set; the rounding thresholds of cells 2j and 2j + 1 are then the high and low halves of
-// mix32(j ^ key) (comp_hash)
+// key ^ (j * 0x9E3779B9) (comp_hash).  Over the sub-steps of one cell the thresholds are as
+// uniform as the keys, which is all the unbiased rounding needs.
 __host__ __device__ inline uint32_t comp_step_key(uint32_t k, int s) {
   return mix32(k * 2u + uint32_t(s));
 }
 // rounding hash of cell i (its threshold in the high 16 bits)
+constexpr uint32_t kCompPairMul = 0x9E3779B9u;
 __host__ __device__ inline uint32_t comp_hash(uint32_t i, uint32_t key) {
-  const uint32_t h = mix32((i >> 1) ^ key);
+  const uint32_t h = key ^ ((i >> 1) * kCompPairMul);
   return (i & 1u) ? h << 16 : h;
 }
 // the code of compensation comp of value c with rounding hash h (h = 0 rounds down: exact for a

@@ -533,7 +533,8 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
   VecA<float, 4> o;
   VecCode<4> qo;
   // i is a multiple of 4: cells i, i+1 share one hash, i+2, i+3 the next (comp_hash)
-  const uint32_t h0 = mix32((uint32_t(i) >> 1) ^ key), h1 = mix32(((uint32_t(i) >> 1) + 1u) ^ key);
+  const uint32_t j = uint32_t(i) >> 1;
+  const uint32_t h0 = key ^ (j * kCompPairMul), h1 = key ^ ((j + 1u) * kCompPairMul);
   const uint32_t h[4] = {h0, h0 << 16, h1, h1 << 16};
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
