@@ -8,11 +8,11 @@
 // -ffp-contract=off, and HIP's division and square root are IEEE correctly rounded on gfx950.
 //
 // Kernels in this file:
-//   unit_phase_kernel, classify_kernel  once per phase-field sweep: per-cell flag of "bulk" cells
-//                          whose whole stencil sits in one phase (exact reduction of the formula,
-//                          see below)
-//   diffuse_bulk_kernel    the default sub-step for large domains: column-marching, U planes in
-//                          flight per lane, bulk cells skip the partition and weights
+//   unit_phase(4)_kernel, classify(4)_kernel  once per phase-field sweep: per-cell flag of "bulk"
+//                          cells whose whole stencil sits in one phase (exact reduction of the
+//                          formula, see below) and the bulk cells per phase
+//   diffuse_bulk_kernel    column-marching, U planes in flight per lane, bulk cells skip the
+//                          partition and weights (used when nx is not a multiple of the run length)
 //   diffuse_linear_kernel  the same arithmetic, one thread per cell in storage order (small domains)
 //   diffuse_bulk_vec_kernel    bulk cells only, 16-byte runs per lane, U planes per batch
 //   diffuse_bulk_march_kernel  the bulk update of the most common bulk phase for every cell, no
@@ -24,6 +24,7 @@
 //                          whose flag is not its bulk phase)
 //   diffuse_kernel         full formula for every cell, LDS-tiled (selected with
 //                          PFAMD_DIFF_KERNEL=lds; kept as an independent path for the parity tests)
+//   general_count/scan/scatter_kernel  the storage-order list of the cells a bulk kernel leaves
 //
 // diffuse_kernel layout: a workgroup owns a 64 x TY column tile and marches through a z-chunk, one
 // plane per iteration.  It has TY + 3 wavefronts: TY interior rows, the rows above and below the tile, and
