@@ -299,7 +299,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU halo design is described in DESIGN.md)"}))
+                                   "(multi-GPU halo exchange is not built)"}))
         return 2
     from phasefield_amd import Domain, Model, PF_MIXED, configs
 

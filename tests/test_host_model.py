@@ -88,7 +88,7 @@ def test_reference_example_parameters_match_oracle(ref_example):
 
 def test_workload_parameters(tmp_path):
     """The named workloads: 200 carbon sub-steps per step in 2D and 3D, dt values of the
-    physics (SURVEY-free closed forms: pfm_dt = dx^2/dims*0.2/(M sigma))."""
+    physics (closed forms: pfm_dt = dx^2/dims*0.2/(M sigma))."""
     for wl, dims in (("fec_2d_256", 2), ("fec_3d_256", 3)):
         m = Model(configs.write_inputs(str(tmp_path / wl), wl))
         p = m.update_parameter()
