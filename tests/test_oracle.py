@@ -525,3 +525,136 @@ def test_flat_interface_velocity(params3d, dG):
             vols.append(_grain_profile(o.get_state(), 100001, nx, ny, K).sum())
     v = (vols[1] - vols[0]) * r.dx / 2.0 / (400 * r.pfm_dt)
     assert 0.9 < v / (M * dG) < 1.0, v / (M * dG)
+
+
+# ---- three behaviours of the reference's step that the checks above leave open ------------------
+
+@pytest.mark.parametrize("phi1", [0.3, 0.55])
+def test_two_phase_mixture_fourier_decay(params3d, phi1):
+    """The diffusion weight is sqrt(phi_p(nb) * phi_p) (solver.cpp:363-364).  On a uniform two-phase
+    mixture (phi_1, phi_2) = (a, 1 - a) with partition tables k(i,j) = 1, c(i,j) = 0, every phase
+    carries the cell's concentration and the weight of phase p is sqrt(a^2) = a exactly, so the
+    sub-step is a one-phase sub-step with D_eff = a D_1 + (1 - a) D_2: a Fourier mode decays by
+    g = 1 - 4 r_eff sum sin^2(pi m/N) per sub-step.  A weight without the square root gives
+    a^2 D_1 + (1 - a)^2 D_2, and a swapped phase index (1 - a) D_1 + a D_2; both miss by far more
+    than the tolerance."""
+    nx, ny, nz, K = 24, 12, 8, 4
+    mode = (2, 1, 1)
+    p = oracle.as_orc_params(params3d)
+    for i in range(3):
+        for j in range(3):
+            for s in range(2):
+                p.part_k[s][i][j], p.part_c[s][i][j] = 1.0, 0.0
+    D1, D2 = 60.0, 100.0
+    p.diffusivity[0][0], p.diffusivity[0][1], p.diffusivity[0][2] = D1, D2, 7.0
+    N = nx * ny * nz
+    ids = np.zeros((N, K), np.int32)
+    val = np.zeros((N, K))
+    hi, lo = (100001, phi1), (200001, 1 - phi1)
+    a, b = (hi, lo) if phi1 >= 0.5 else (lo, hi)
+    ids[:, 0], val[:, 0] = a
+    ids[:, 1], val[:, 1] = b
+    phase = np.zeros((N, 3))
+    phase[:, 0], phase[:, 1] = phi1, 1 - phi1
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    arg = 2 * np.pi * (mode[0] * x / nx + mode[1] * y / ny + mode[2] * z / nz)
+    c0 = (0.4 + 0.1 * np.cos(arg)).transpose(2, 1, 0).reshape(-1)
+    od = oracle.OracleDomain(nx, ny, nz, K, 1)
+    od.set_params(p)
+    od.set_state(ids.reshape(-1), val.reshape(-1), phase.reshape(-1), c0)
+    dt_d, nsub = 4.0e-6, 40
+    od.diffuse(0, nsub, dt_d)
+    got = od.get_state()["conc"]
+    s2 = sum(np.sin(np.pi * m / n) ** 2 for m, n in zip(mode, (nx, ny, nz)))
+
+    def decayed(deff):
+        g = 1.0 - 4.0 * deff * dt_d / (p.dx * p.dx) * s2
+        return 0.4 + (c0 - 0.4) * g ** nsub
+
+    err = np.max(np.abs(got - decayed(phi1 * D1 + (1 - phi1) * D2)))
+    assert err < 1e-13, err
+    for wrong in (phi1 ** 2 * D1 + (1 - phi1) ** 2 * D2, (1 - phi1) * D1 + phi1 * D2):
+        assert np.max(np.abs(got - decayed(wrong))) > 1e3 * max(err, 1e-16)
+
+
+def _uniform_state(N, K, slots):
+    ids = np.zeros((N, K), np.int32)
+    val = np.zeros((N, K))
+    for s, (g, v) in enumerate(slots):
+        ids[:, s], val[:, s] = g, v
+    phase = np.zeros((N, 3))
+    for g, v in slots:
+        phase[:, 0 if g < 200000 else 1] += v
+    return ids.reshape(-1), val.reshape(-1), phase.reshape(-1)
+
+
+@pytest.mark.parametrize("case", ["tie_first", "tie_behind_max"])
+def test_sort_ties_follow_first_max_and_swap(params3d, case):
+    """Sort_xyz (solver.cpp:426-432) is a selection sort that moves the first maximum of D[i..10]
+    to position i by a swap (solver.cpp:273), so ties are not kept in union order in general.
+    Every cell of a uniform domain holds two ferrite grains A, C with equal phi (their updates are
+    the same operations on the same operands, so they tie bitwise) and an austenite grain B.
+    - tie_first: B is smallest after the update and comes last in the union; the tie stays in
+      union order A, C.
+    - tie_behind_max: B leads after the update but comes last in the union (A, C, B); the first
+      pass swaps B with A, giving B, C, A.  A stable sort or a last-maximum rule gives B, A, C."""
+    nx, ny, nz, K = 4, 4, 4, 4
+    N = nx * ny * nz
+    A, C, B = 200001, 200002, 100001
+    if case == "tie_first":
+        slots, conc = [(A, 0.4), (C, 0.4), (B, 0.2)], 0.02
+    else:
+        slots, conc = [(A, 0.34), (C, 0.34), (B, 0.32)], 0.8
+    ids, val, phase = _uniform_state(N, K, slots)
+    od = oracle.OracleDomain(nx, ny, nz, K, 1)
+    od.set_params(params3d)
+    od.set_state(ids, val, phase, np.full(N, conc))
+    od.sweep()
+    s = od.get_state()
+    oi = s["index"].reshape(N, K)
+    ov = s["value"].reshape(N, K)
+    if case == "tie_first":
+        assert np.all(oi[:, :3] == [A, C, B])
+        assert np.all(ov[:, 0] == ov[:, 1]) and np.all(ov[:, 2] < ov[:, 0])
+    else:
+        assert np.all(oi[:, :3] == [B, C, A])
+        assert np.all(ov[:, 1] == ov[:, 2]) and np.all(ov[:, 0] > ov[:, 1])
+
+
+@pytest.mark.parametrize("own", [6, 5])
+def test_union_cap_drops_the_eleventh_grain(params3d, own):
+    """The active-set union holds at most 10 entries; a neighbour grain met when it is full is
+    dropped silently (solver.cpp:197-202).  K = 6: the centre cell owns `own` ferrite grains, its -x
+    neighbour four more, and every other cell of the domain holds grain G alone, so G reaches the
+    union through five faces, after those ten (own = 6) or as the tenth entry (own = 5).  As the
+    tenth entry G takes the largest neighbour sum and enters the centre's output at slot 1; as an
+    eleventh it never enters it."""
+    nx, ny, nz, K = 5, 5, 5, 6
+    N = nx * ny * nz
+    G = 200099
+    ids = np.zeros((N, K), np.int32)
+    val = np.zeros((N, K))
+    ids[:, 0], val[:, 0] = G, 1.0
+    c = 2 + nx * (2 + ny * 2)
+    xm = c - 1
+    ids[c], val[c] = 0, 0.0
+    for s in range(own):
+        ids[c, s], val[c, s] = 200001 + s, 1.0 / own
+    ids[xm], val[xm] = 0, 0.0
+    for s in range(4):
+        ids[xm, s], val[xm, s] = 200011 + s, 0.25
+    phase = np.zeros((N, 3))
+    phase[:, 1] = 1.0
+    od = oracle.OracleDomain(nx, ny, nz, K, 1)
+    od.set_params(params3d)
+    od.set_state(ids.reshape(-1), val.reshape(-1), phase.reshape(-1), np.full(N, 0.02))
+    od.sweep()
+    s = od.get_state()
+    out = s["index"].reshape(N, K)[c]
+    v = s["value"].reshape(N, K)[c]
+    if own == 6:
+        assert G not in out
+        assert set(out[out > 0]) <= set(range(200001, 200007)) | set(range(200011, 200015))
+    else:
+        assert out[0] == G and v[0] == v.max()
+    assert abs(v.sum() - 1.0) < 1e-15
