@@ -256,17 +256,31 @@ int pf_read_state(const char *path, const pf_desc *d, double *time, int64_t *ste
 
 /* ---- compatibility shim: the reference's accelerator API (gpu_solver.h:18-23) --------------- */
 
-/* Returns 1 if a device is usable for an x*y*z domain, else 0 (CPU fallback in the caller). */
-int GPU_Init(const int x, const int y, const int z);
-/* Diffusion-only offload with host pointers, as gpu_solver.cpp:204-218: copies value_phase[3N]
- * and total_conc[N] in, runs ceil(dt/del_t) partition+diffusion sub-steps, copies total_conc out. */
-void GPU_Diffusion_solver(double *hdiffusivity, double *hpartition_k, double *hpartition_c,
-                          double *hvalue_phase, double *htotal_conc, int x, int y, int z, double dt,
-                          double dx);
-void GPU_Cleanup(void);
+/* C entry points of the shim (C callers, FFI).  C++ callers use the reference's own names below. */
+/* Returns 1 if a device is usable for an x*y*z domain, else 0 (CPU fallback in the caller):
+ * GPU_Init, gpu_solver.cpp:26-99. */
+int pf_gpu_init(int x, int y, int z);
+/* Diffusion-only offload with host pointers, as GPU_Diffusion_solver (gpu_solver.cpp:204-218):
+ * copies value_phase[3N] and total_conc[N] in, runs ceil(dt/del_t) partition+diffusion sub-steps,
+ * copies total_conc out.  Returns PF_OK or an error status (the reference returns nothing). */
+int pf_gpu_diffusion_solver(const double *hdiffusivity, const double *hpartition_k,
+                            const double *hpartition_c, const double *hvalue_phase,
+                            double *htotal_conc, int x, int y, int z, double dt, double dx);
+/* Frees the shim's domain: GPU_Cleanup, gpu_solver.cpp:223-239. */
+void pf_gpu_cleanup(void);
 
 #ifdef __cplusplus
 }
+
+/* The reference's accelerator API with the reference's C++ linkage, exactly as
+ * C++-openmp-cuda/src/gpu_solver.h:18-23 declares it (real = double, real.h), so that the
+ * reference's phase_field.cpp, which includes gpu_solver.h (phase_field.h:14), links against this
+ * library unchanged.  Each forwards to the pf_gpu_* entry point above. */
+int GPU_Init(const int x, const int y, const int z);
+void GPU_Diffusion_solver(double *hdiffusivity, double *hpartition_k, double *hpartition_c,
+                          double *hvalue_phase, double *htotal_conc, int x, int y, int z,
+                          double dt, double dx);
+void GPU_Cleanup();
 #endif
 
 #endif /* PHASEFIELD_AMD_H */

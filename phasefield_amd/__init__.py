@@ -463,7 +463,7 @@ def read_vti_raw(path: str) -> dict:
 
 # ---- the reference's accelerator API (gpu_solver.h:18-23) -----------------------------------
 def gpu_init(x: int, y: int, z: int) -> int:
-    return lib().GPU_Init(x, y, z)
+    return lib().pf_gpu_init(x, y, z)
 
 
 def gpu_diffusion_solver(diffusivity, partition_k, partition_c, value_phase, total_conc,
@@ -472,9 +472,11 @@ def gpu_diffusion_solver(diffusivity, partition_k, partition_c, value_phase, tot
     arrs = [np.ascontiguousarray(a, np.float64) for a in (diffusivity, partition_k, partition_c,
                                                           value_phase)]
     assert total_conc.dtype == np.float64 and total_conc.flags.c_contiguous
-    lib().GPU_Diffusion_solver(*[_dp(a) for a in arrs], _dp(total_conc), x, y, z, float(dt),
-                               float(dx))
+    rc = lib().pf_gpu_diffusion_solver(*[_dp(a) for a in arrs], _dp(total_conc), x, y, z,
+                                       float(dt), float(dx))
+    if rc != 0:
+        raise PhaseFieldError(rc, "GPU_Diffusion_solver failed (see stderr)")
 
 
 def gpu_cleanup() -> None:
-    lib().GPU_Cleanup()
+    lib().pf_gpu_cleanup()

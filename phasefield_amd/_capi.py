@@ -119,10 +119,10 @@ SIGNATURES = {
                             C.POINTER(PfHostState)]),
     "pf_read_state": (_I, [C.c_char_p, C.POINTER(PfDesc), C.POINTER(_D), C.POINTER(C.c_int64),
                            C.POINTER(PfHostState)]),
-    "GPU_Init": (_I, [_I, _I, _I]),
-    "GPU_Diffusion_solver": (None, [C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D),
-                                    C.POINTER(_D), _I, _I, _I, _D, _D]),
-    "GPU_Cleanup": (None, []),
+    "pf_gpu_init": (_I, [_I, _I, _I]),
+    "pf_gpu_diffusion_solver": (_I, [C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D),
+                                     C.POINTER(_D), _I, _I, _I, _D, _D]),
+    "pf_gpu_cleanup": (None, []),
 }
 
 _lib = None

@@ -1328,11 +1328,12 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
 // ---- the reference's accelerator API (gpu_solver.h:18-23) ------------------------------------
 // One process-wide domain, as the reference keeps static device buffers.  value_phase carries all
 // three phases; the stencil always has six faces (the z faces of a one-cell-thick domain meet the
-// cell itself and carry no flux), as in Diff3D (gpu_solver.cpp:160-188).
+// cell itself and carry no flux), as in Diff3D (gpu_solver.cpp:160-188).  These are the C entry
+// points; gpu_solver_compat.cpp exports the reference's C++-linkage names over them.
 static pf_handle *g_compat = nullptr;
 static int g_compat_dims = 3;
 
-int GPU_Init(const int x, const int y, const int z) {
+int pf_gpu_init(int x, int y, int z) {
   if (g_compat) {
     pf_destroy(g_compat);
     g_compat = nullptr;
@@ -1357,14 +1358,25 @@ int GPU_Init(const int x, const int y, const int z) {
   return 1;
 }
 
-void GPU_Diffusion_solver(double *hdiffusivity, double *hpartition_k, double *hpartition_c,
-                          double *hvalue_phase, double *htotal_conc, int x, int y, int z,
-                          double dt, double dx) {
+int pf_gpu_diffusion_solver(const double *hdiffusivity, const double *hpartition_k,
+                            const double *hpartition_c, const double *hvalue_phase,
+                            double *htotal_conc, int x, int y, int z, double dt, double dx) {
   pf_handle *h = g_compat;
   if (!h || h->d.nx != x || h->d.ny != y || h->d.nz != z) {
-    std::fprintf(stderr, "GPU_Diffusion_solver: GPU_Init was not called for this shape\n");
-    return;
+    // The reference's Update (phase_field.cpp:289-290) calls this without a GPU_Init (only
+    // cudatest.cpp:10 calls it), so the domain is set up here on first use or a new shape.
+    if (pf_gpu_init(x, y, z) != 1) {
+      std::fprintf(stderr, "GPU_Diffusion_solver: no usable device for a %dx%dx%d domain\n", x, y,
+                   z);
+      return PF_ERR_NODEVICE;
+    }
+    h = g_compat;
   }
+  if (!hdiffusivity || !hpartition_k || !hpartition_c || !hvalue_phase || !htotal_conc) {
+    std::fprintf(stderr, "GPU_Diffusion_solver: null pointer\n");
+    return PF_ERR_INVALID;
+  }
+  hipSetDevice(h->d.device);
   pf_params p{};
   p.dx = dx;
   p.eta = dx;
@@ -1387,32 +1399,36 @@ void GPU_Diffusion_solver(double *hdiffusivity, double *hpartition_k, double *hp
   h->p = p;
   h->have_params = true;
   update_bulk_mask(h);
-  const long N = h->N;
-  std::vector<int32_t> idx(size_t(N) * 4, 0);
-  std::vector<double> val(size_t(N) * 4, 0.0);
-  pf_host_state s{};
-  s.index = idx.data();
-  s.value = val.data();
-  s.phase = hvalue_phase;
-  s.conc = htotal_conc;
-  if (pf_upload_state(h, &s) != PF_OK) {
-    std::fprintf(stderr, "GPU_Diffusion_solver: %s\n", h->err.c_str());
-    return;
-  }
+  // only what the sub-steps read: value_phase [N][3] and total_conc [N], both already in the
+  // device layout (gpu_solver.cpp:106-129 copies the same two arrays plus the tables)
+  const size_t N = size_t(h->N);
+  int rc = PF_OK;
+  auto report = [&](int code, const char *what, hipError_t e) {
+    std::fprintf(stderr, "GPU_Diffusion_solver: %s: %s\n", what, hipGetErrorString(e));
+    return fail(h, code, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e = hipMemcpyAsync(h->pfrac, hvalue_phase, N * 3 * 8, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h->conc[0][0], htotal_conc, N * 8, hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) return report(PF_ERR_HIP, "upload", e);
+  h->ccur[0] = 0;
+  h->csub[0] = 0;
+  h->have_state = true;
+  h->flag_valid = false;
   h->NF = 6;  // Diff3D always sums six faces
-  const int rc = diffuse_any(h, 0, num_loop, dt_dxdx);
+  rc = diffuse_any(h, 0, num_loop, dt_dxdx);
   h->NF = 2 * h->dims;
   if (rc != PF_OK) {
     std::fprintf(stderr, "GPU_Diffusion_solver: %s\n", h->err.c_str());
-    return;
+    return rc;
   }
-  if (hipStreamSynchronize(h->stream) != hipSuccess ||
-      hipMemcpy(htotal_conc, h->conc[0][h->ccur[0]], size_t(N) * 8, hipMemcpyDeviceToHost) !=
-          hipSuccess)
-    std::fprintf(stderr, "GPU_Diffusion_solver: copy back failed\n");
+  e = hipMemcpyAsync(htotal_conc, h->conc[0][h->ccur[0]], N * 8, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return report(PF_ERR_HIP, "copy back", e);
+  return PF_OK;
 }
 
-void GPU_Cleanup(void) {
+void pf_gpu_cleanup(void) {
   if (g_compat) pf_destroy(g_compat);
   g_compat = nullptr;
 }

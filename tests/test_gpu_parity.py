@@ -368,6 +368,40 @@ def test_compat_shim_matches_oracle(base_params, rng):
     assert np.array_equal(c, o.get_state()["conc"])
 
 
+def test_reference_cxx_caller_matches_oracle(base_params, rng, tmp_path):
+    """A C++ program that declares GPU_Init / GPU_Diffusion_solver / GPU_Cleanup exactly as the
+    reference's gpu_solver.h:18-23 does (C++ linkage, no header of this library), linked with g++
+    against libpfamd.so, gives the oracle's concentrations bitwise (2D and 3D shapes)."""
+    import subprocess
+    banner("reference-shaped C++ caller")
+    exe = helpers.build_reference_shim_program(str(tmp_path))
+    p = base_params
+    D = np.array([p.diffusivity[0][i] for i in range(3)])
+    pk = np.array([[p.part_k[0][i][j] for j in range(3)] for i in range(3)])
+    pc = np.array([[p.part_c[0][i][j] for j in range(3)] for i in range(3)])
+    for (nx, ny, nz) in ((36, 20, 1), (24, 18, 10)):
+        K = 4
+        ids, val, phase, conc = helpers.random_state(rng, nx, ny, nz, K)
+        dt, dx = p.pfm_dt, p.dx
+        inp, out = tmp_path / f"in{nz}.bin", tmp_path / f"out{nz}.bin"
+        helpers.write_shim_input(str(inp), nx, ny, nz, dt, dx, D, pk.T.reshape(-1),
+                                 pc.T.reshape(-1), phase, conc)
+        res = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stdout + res.stderr
+        got = np.fromfile(str(out), np.float64)
+        dims = 3 if nz > 1 else 2
+        del_t = dx * dx / D.max() / dims * float(np.float32(0.2))
+        nloop = int(np.ceil(dt / del_t))
+        q = oracle.as_orc_params(p)
+        q.dims = 3  # Diff3D sums six faces; the z faces of a one-cell-thick domain carry no flux
+        o = oracle.OracleDomain(nx, ny, nz, K, 1)
+        o.set_params(q)
+        o.set_state(ids, val, phase, conc)
+        o.diffuse_factor(0, nloop, dt / nloop / (dx * dx))
+        assert np.array_equal(got, o.get_state()["conc"]), (nx, ny, nz)
+
+
 @pytest.mark.slow
 def test_fec_3d_256_steps_bitwise(tmp_path):
     """The 256^3 Fe-C parity workload at full size (128 grains, 200 nuclei, K = 4): the device-built
