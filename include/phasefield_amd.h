@@ -140,8 +140,9 @@ int pf_synchronize(pf_handle *h);
  * classifies bulk cells), 1 = phase-field sweep. */
 int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches);
 int pf_kernel_timing_reset(pf_handle *h, int enable);
-/* Size in bytes of one device-side checkpoint; save/restore copy the whole state (device to
- * device when dst/src is a device pointer from pf_checkpoint_alloc, else host memory). */
+/* Size in bytes of one checkpoint of this handle.  save/restore copy the whole state, the mixed
+ * form's sub-step counters and the grain-code tables to / from host memory of that size; restore
+ * rejects a blob whose header names another domain shape, form or size. */
 int pf_checkpoint_size(const pf_handle *h, size_t *bytes);
 int pf_checkpoint_save(pf_handle *h, void *host_dst);
 int pf_checkpoint_restore(pf_handle *h, const void *host_src);

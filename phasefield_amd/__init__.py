@@ -209,6 +209,12 @@ class Domain:
         return buf.raw
 
     def restore(self, blob: bytes) -> None:
+        n = C.c_size_t()
+        _check(lib().pf_checkpoint_size(self._h, C.byref(n)), self._h, "restore")
+        if len(blob) != n.value:
+            raise PhaseFieldError(_capi.PF_ERR_INVALID,
+                                  f"restore: checkpoint of {len(blob)} bytes, this domain's take "
+                                  f"{n.value}")
         buf = C.create_string_buffer(blob, len(blob))
         _check(lib().pf_checkpoint_restore(self._h, buf), self._h, "restore")
 

@@ -100,7 +100,20 @@ def main_text(wl: dict, outdir: str, thermo_name: str) -> str:
                     "mn_D0_1": 1e-6, "mn_D0_2": 1e-4, "mn_D0_3": 0.0, "mn_substeps": 1})
     ext.update(wl.get("extra", {}))
     ext_lines = "\n".join(f"{k}={v}" for k, v in ext.items())
-    M, sig = "1e12 0", "0.5e-12"
+    # optional overrides: an Arrhenius mobility / diffusivities (prefactor, Q in J/mol) and a
+    # piecewise-linear temperature path [(time, T), ...]
+    def pair(v):
+        return f"{float(v[0])!r} {float(v[1])!r}"
+
+    M = pair(wl["mobility"]) if "mobility" in wl else "1e12 0"
+    sig = "0.5e-12"
+    d_gam = (pair(wl["diff_gamma"]) if "diff_gamma" in wl
+             else f"{wl.get('D_gamma', 0.5)} 0")
+    d_alp = (pair(wl["diff_alpha"]) if "diff_alpha" in wl
+             else f"{wl.get('D_alpha', 100.0)} 0")
+    t_lines = (f"2\n0 {T_K}\n1e6 {T_K}" if "temperature" not in wl else
+               "\n".join([str(len(wl["temperature"]))] +
+                         [pair(tT) for tT in wl["temperature"]]))
     return f"""OUTPUT NAME?
 {outdir}
 GRID SIZE?
@@ -120,9 +133,7 @@ VOLUME FRACTION OF SECOND PHASE (<1):
 solute concentration in each phase (austenite, ferrite, placeholder)?
 0.8 0.02 0.
  temperature [K]?
-2
-0 {T_K}
-1e6 {T_K}
+{t_lines}
  rolling reduction(<1):
 {wl.get('reduction', 0.)}
  Number of iterations for initialisation:
@@ -158,9 +169,9 @@ austenite nuclei number in ferrite?
  placeholder-placeholder interface energy[J/um^2]?
  0
  austenite diffusivity D0 [um^2/s], Activation energy Q[J]?
- {wl.get('D_gamma', 0.5)} 0
+ {d_gam}
  ferrite diffusivity D0,Activation energy Q?
- {wl.get('D_alpha', 100.0)} 0
+ {d_alp}
  placeholder diffusivity D0,Activation energy Q?
  0 0
 thermodynamic file?

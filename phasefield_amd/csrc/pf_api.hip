@@ -1045,11 +1045,25 @@ int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches
 }
 
 // ---- checkpoints (host memory, raw device layout) -----------------------------------------------
+// Blob: a 64-byte header (magic, pf_desc, the mixed form's sub-step counters, the blob's size), the
+// grain-code tables (per phase 1..3: how many entries, then up to kCodeCap reference ids, so that a
+// restore into any handle maps codes back to the same ids), then the device buffers of the current
+// state in the device layout.
+namespace {
+constexpr char kCkptMagic[9] = "PFCKPT03";
+constexpr size_t kCodeCap = 16384;  // serials 0..16383 of a 14-bit grain code
+constexpr size_t kCkptHeader = 64;
+constexpr size_t kCkptTables = 4 * sizeof(int32_t) + 3 * kCodeCap * sizeof(int32_t);
+size_t checkpoint_bytes(const pf_handle *h) {
+  const size_t N = size_t(h->N);
+  return kCkptHeader + kCkptTables + N * h->KI * 2 + N * h->d.depth * h->tsz +
+         N * h->d.nphase * h->tsz + N * h->csz * h->d.nsolute;
+}
+}  // namespace
+
 int pf_checkpoint_size(const pf_handle *h, size_t *bytes) {
   if (!h || !bytes) return PF_ERR_INVALID;
-  const size_t N = size_t(h->N);
-  *bytes = 64 + N * h->KI * 2 + N * h->d.depth * h->tsz + N * h->d.nphase * h->tsz +
-           N * h->csz * h->d.nsolute;
+  *bytes = checkpoint_bytes(h);
   return PF_OK;
 }
 
@@ -1058,13 +1072,27 @@ int pf_checkpoint_save(pf_handle *h, void *dst) {
   if (!h->have_state) return fail(h, PF_ERR_STATE, "no state on the device");
   hipSetDevice(h->d.device);
   const size_t N = size_t(h->N);
+  const uint64_t total = checkpoint_bytes(h);
   char *p = static_cast<char *>(dst);
-  static_assert(8 + sizeof(pf_desc) + sizeof(h->csub) <= 64, "checkpoint header");
-  std::memset(p, 0, 64);
-  std::memcpy(p, "PFCKPT02", 8);
+  static_assert(8 + sizeof(pf_desc) + sizeof(pf_handle::csub) + sizeof(uint64_t) <= kCkptHeader,
+                "checkpoint header");
+  std::memset(p, 0, kCkptHeader + kCkptTables);
+  std::memcpy(p, kCkptMagic, 8);
   std::memcpy(p + 8, &h->d, sizeof(pf_desc));
   std::memcpy(p + 8 + sizeof(pf_desc), h->csub, sizeof(h->csub));
-  p += 64;
+  std::memcpy(p + 8 + sizeof(pf_desc) + sizeof(h->csub), &total, sizeof(total));
+  p += kCkptHeader;
+  int32_t count[4] = {0, 0, 0, 0};
+  for (int ph = 1; ph <= 3; ++ph) {
+    if (h->code_to_id[ph].size() > kCodeCap)
+      return fail(h, PF_ERR_RANGE, "grain-code table larger than the checkpoint holds");
+    count[ph] = int32_t(h->code_to_id[ph].size());
+    if (count[ph])
+      std::memcpy(p + sizeof(count) + (ph - 1) * kCodeCap * sizeof(int32_t),
+                  h->code_to_id[ph].data(), count[ph] * sizeof(int32_t));
+  }
+  std::memcpy(p, count, sizeof(count));
+  p += kCkptTables;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(p, h->id[h->cur], N * h->KI * 2, hipMemcpyDeviceToHost));
   p += N * h->KI * 2;
@@ -1085,12 +1113,28 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
   const size_t N = size_t(h->N);
   const char *p = static_cast<const char *>(src);
   pf_desc d;
+  uint64_t total = 0;
   std::memcpy(&d, p + 8, sizeof(d));
-  if (std::memcmp(p, "PFCKPT02", 8) != 0 || std::memcmp(&d, &h->d, sizeof(d)) != 0)
+  std::memcpy(&total, p + 8 + sizeof(pf_desc) + sizeof(h->csub), sizeof(total));
+  if (std::memcmp(p, kCkptMagic, 8) != 0 || std::memcmp(&d, &h->d, sizeof(d)) != 0 ||
+      total != checkpoint_bytes(h))
     return fail(h, PF_ERR_INVALID, "checkpoint does not match this domain");
   uint32_t csub[PF_MAX_SOLUTE];
   std::memcpy(csub, p + 8 + sizeof(pf_desc), sizeof(csub));
-  p += 64;
+  p += kCkptHeader;
+  int32_t count[4];
+  std::memcpy(count, p, sizeof(count));
+  std::vector<int32_t> tables[4];
+  tables[0].assign(1, 0);
+  for (int ph = 1; ph <= 3; ++ph) {
+    if (count[ph] < 0 || size_t(count[ph]) > kCodeCap)
+      return fail(h, PF_ERR_INVALID, "checkpoint holds a bad grain-code table");
+    tables[ph].resize(size_t(count[ph]));
+    if (count[ph])
+      std::memcpy(tables[ph].data(), p + sizeof(count) + (ph - 1) * kCodeCap * sizeof(int32_t),
+                  count[ph] * sizeof(int32_t));
+  }
+  p += kCkptTables;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipMemcpy(h->id[h->cur], p, N * h->KI * 2, hipMemcpyHostToDevice));
   p += N * h->KI * 2;
@@ -1103,6 +1147,7 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
     p += N * h->csz;
   }
   std::memcpy(h->csub, csub, sizeof(csub));
+  for (int ph = 0; ph < 4; ++ph) h->code_to_id[ph] = tables[ph];
   h->have_state = true;
   h->flag_valid = false;
   h->sweep_hist = false;
@@ -1181,6 +1226,31 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
   if (is_mixed(h)) return fail(h, PF_ERR_INVALID, "nucleate on an FP64 domain");
   if (n_nuclei < 0 || n_nuclei > 16383) return fail(h, PF_ERR_RANGE, "bad nucleus count");
   if (!(radius > 0.f)) return fail(h, PF_ERR_INVALID, "radius must be positive");
+  // Nucleus q takes the reference's id 100000 + q (nucleation.cpp:29).  Grains are the same grain
+  // exactly when their ids are equal, so a nucleus whose id a phase-1 grain of the state already
+  // has takes that grain's code, and every other id gets the next free serial; the codes of the
+  // grains already on the device are kept.
+  std::vector<int32_t> table = h->code_to_id[1];
+  if (table.empty()) table.assign(1, 0);
+  std::vector<int> codes(size_t(n_nuclei) + 1, 0);
+  {
+    std::map<int32_t, int> serial;
+    for (size_t s = 1; s < table.size(); ++s) serial[table[s]] = int(s);
+    for (int q = 1; q <= n_nuclei; ++q) {
+      const int32_t gid = 100000 + q;
+      auto it = serial.find(gid);
+      int ser;
+      if (it != serial.end()) {
+        ser = it->second;
+      } else {
+        ser = int(table.size());
+        if (ser > 16383) return fail(h, PF_ERR_RANGE, "more than 16383 grains of phase 1");
+        table.push_back(gid);
+        serial[gid] = ser;
+      }
+      codes[q] = (1 << 14) | ser;
+    }
+  }
   hipSetDevice(h->d.device);
   const int K = h->d.depth, P = h->d.nphase;
   const long N = h->N;
@@ -1232,7 +1302,8 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
     hipLaunchKernelGGL(argmax_stage2, dim3(1), dim3(256), 0, h->stream, pv, pi, nb);
 #define PF_PAINT(KK, PP)                                                                      \
   hipLaunchKernelGGL((nucleus_paint_kernel<KK, PP>), dim3(blocks_for(box)), dim3(256), 0,      \
-                     h->stream, pi, q, radius, R, h->d.nx, h->d.ny, h->d.nz, cntv[0], cntv[1], \
+                     h->stream, pi, codes[q], radius, R, h->d.nx, h->d.ny, h->d.nz, cntv[0],  \
+                     cntv[1],                                                                 \
                      cntv[2], h->id[h->cur], phi, pf, flag, mark)
     if (K == 4 && P == 2) PF_PAINT(4, 2);
     else if (K == 4) PF_PAINT(4, 3);
@@ -1250,8 +1321,7 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   cleanup();
   if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("nucleate: ") + hipGetErrorString(e));
-  h->code_to_id[1].assign(1, 0);
-  for (int q = 1; q <= n_nuclei; ++q) h->code_to_id[1].push_back(100000 + q);
+  h->code_to_id[1] = table;
   h->flag_valid = false;
   h->sweep_hist = false;
   return PF_OK;

@@ -147,7 +147,7 @@ __global__ void argmax_stage1(const float *__restrict__ flag, long ncell, long p
                               float *__restrict__ pv, long long *__restrict__ pi);
 __global__ void argmax_stage2(float *__restrict__ pv, long long *__restrict__ pi, int nparts);
 template <int K, int P>
-__global__ void nucleus_paint_kernel(const long long *__restrict__ center, int q, float radius,
+__global__ void nucleus_paint_kernel(const long long *__restrict__ center, int code, float radius,
                                      int R, int nx, int ny, int nz, int cx_n, int cy_n, int cz_n,
                                      uint16_t *__restrict__ id, double *__restrict__ phi,
                                      double *__restrict__ pfrac, float *__restrict__ flag,

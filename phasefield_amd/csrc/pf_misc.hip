@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void argmax_stage2(float *__restrict__ pv,
 // affected cell; along an axis shorter than 2R+1 the whole axis is scanned.
 template <int K, int P>
 __global__ __launch_bounds__(256) void nucleus_paint_kernel(
-    const long long *__restrict__ center, int q, float radius, int R, int nx, int ny, int nz,
+    const long long *__restrict__ center, int code, float radius, int R, int nx, int ny, int nz,
     int cx_n, int cy_n, int cz_n, uint16_t *__restrict__ id, double *__restrict__ phi,
     double *__restrict__ pfrac, float *__restrict__ flag, uint8_t *__restrict__ mark) {
   const long t = long(blockIdx.x) * 256 + threadIdx.x;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void nucleus_paint_kernel(
     uint32_t g[K];
 #pragma unroll
     for (int s = 0; s < K; ++s) g[s] = 0;
-    g[0] = (1u << 14) | uint32_t(q);
+    g[0] = uint32_t(code) & 0xffffu;
     store_ids<K>(id, cell, g);
 #pragma unroll
     for (int s = 0; s < K; ++s) phi[cell * K + s] = s == 0 ? 1.0 : 0.0;
