@@ -7,16 +7,27 @@ manganese update for two-solute workloads), with the per-step parameter update o
 Default workload: 512^3 Fe-C-Mn, 512 Voronoi ferrite grains + 1000 austenite nuclei, K = 4.
 `value` is the mixed form (FP32 storage/arithmetic, Kahan-compensated concentration, FP64
 pruning/renormalisation); the FP64 form runs the same steps from the same seeded state and is
-reported beside it with the mixed form's error against it.  The state is resident in HBM before
-the timed region; the region is bracketed by stream synchronisation.
-The roofline entry is for the dominant kernel (the diffusion sub-step): algorithmic bytes per
-launch (read phase fractions + stored concentration, write concentration) over the mean launch
-time from HIP events on the stepping stream.  cpu_baseline times the oracle's OpenMP path
-(oracle/, a restatement of the reference's CPU loop) on the box's host cores on a bounded sample.
+reported beside it.  The state is resident in HBM before the timed region; the region is bracketed
+by stream synchronisation.
+
+Roofline of the dominant kernel, the plane-marching diffusion kernel (about 80 % of a step): the
+bytes it must move per launch (every cell's stored concentration read once and written once: FP64
+8 + 8 B, mixed 4 + 1 B each way) over its mean launch time from HIP event pairs around a sample of
+its launches inside the timed region, on the stream it runs on.  `substep` adds the listed pass
+(the cells next to a phase boundary, full formula) with the list length read from the device, and
+the gaps between the two launches.  `model_8d` states the same run in the compulsory-traffic model
+of the scope document (every cell reads value_phase; 24 / 32 B per cell per sub-step).
+After the timed steps both forms continue to step 1000 from the same state and the mixed form's
+error against FP64 is reported along the way.  cpu_baseline times the oracle's OpenMP path
+(oracle/, a restatement of the reference's CPU loop) on the host cores: a bounded sample of this
+workload, and the reference's 2D 256^2 case in full.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
+import hashlib
 import json
 import os
 import sys
@@ -28,6 +39,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s FP32-mix vs FP64 at stated error; %HBM roofline; adjoint/fwd"
+# the sources of the diffusion kernels whose HBM traffic profiles/ records (a changed source makes
+# the recorded traffic stale)
+TRAFFIC_SOURCES = ("phasefield_amd/csrc/pf_diffuse.hip", "phasefield_amd/csrc/pf_diffuse_common.h",
+                   "phasefield_amd/csrc/pf_device.h")
 
 
 def log(msg: str) -> None:
@@ -54,18 +69,44 @@ def setup_fp64(m, p):
     return g
 
 
-def alg_bytes_per_substep(mixed: bool, nphase: int) -> int:
-    """Compulsory bytes of one carbon diffusion sub-step per cell: read value_phase (P values) and
-    the stored concentration, write the concentration (FP64: 8-byte values; mixed: 4-byte phase
-    fractions and a 5-byte concentration = FP32 value + one-byte Kahan compensation code)."""
-    if mixed:
-        return nphase * 4 + 5 + 5
-    return nphase * 8 + 8 + 8
+# ---- byte models ----------------------------------------------------------------------------------
+def march_bytes_per_cell(mixed: bool) -> int:
+    """Bytes the plane-marching kernel must move per cell and launch: the stored concentration read
+    once and written once (FP64 8 B each way; mixed FP32 value + one-byte compensation code)."""
+    return 5 + 5 if mixed else 8 + 8
+
+
+def listed_bytes_per_cell(mixed: bool, nphase: int) -> int:
+    """Bytes of one listed cell's full-formula update with no reuse between cells: its own and its six
+    neighbours' stored concentration and value_phase read, its concentration written."""
+    b_c, b_phi = (5, 4) if mixed else (8, 8)
+    return 7 * (b_c + nphase * b_phi) + b_c
+
+
+def model_8d_bytes_per_cell_step(mixed: bool, nphase: int, nsolute: int, n_sub: int) -> int:
+    """The scope document's compulsory model for one whole step: a sweep (K = 4 slots of int16 id
+    + phi, read and written, value_phase read and written, each solute's concentration read) and,
+    per sub-step, value_phase + concentration read and the concentration written (mixed: FP32
+    value + FP32 Kahan term), manganese once per step."""
+    b_phi, b_c, K = (4, 8, 4) if mixed else (8, 8, 4)
+    sweep = 2 * K * (2 + b_phi) + 2 * nphase * b_phi + nsolute * b_c
+    sub = nphase * b_phi + 2 * b_c
+    return sweep + n_sub * sub + (nsolute - 1) * sub
+
+
+def kernel_source_hash() -> str:
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def measured_traffic(workload: str) -> dict:
-    """HBM bytes per carbon sub-step measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a
-    plain copy of known size), from the `TRAFFIC {...}` line of the summaries in profiles/."""
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (each scaled by what
+    it reads for a plain copy of known size), from the `TRAFFIC {...}` line of a summary in
+    profiles/ for this workload.  The line carries the hash of the diffusion kernels' sources it was
+    measured with; `stale` is set when the sources have changed since."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return {}
@@ -77,17 +118,16 @@ def measured_traffic(workload: str) -> dict:
                 if line.startswith("TRAFFIC "):
                     rec = json.loads(line[len("TRAFFIC "):])
                     if rec.get("workload") == workload:
+                        rec["file"] = f"profiles/{name}"
+                        rec["stale"] = rec.get("sources") != kernel_source_hash()
                         return rec
     return {}
 
 
-def diffusion_kernel_name(mixed: bool, nphase: int, noflux: bool) -> str:
-    """The sub-step kernels the library launches for a 3D domain whose nx is a multiple of the
-    16-byte run (tile rows 8 or 16, picked by timing at the handle's first sub-step)."""
+def march_kernel_name(mixed: bool, nphase: int, noflux: bool, rows: int) -> str:
     form = "FormMixed" if mixed else "FormF64"
-    nof = "true" if noflux else "false"
-    return (f"diffuse_bulk_march_kernel<{form},{nphase},6,{nof},8|16,{4 if mixed else 2}> + "
-            f"diffuse_general_kernel<{form},{nphase},6,{nof}> (one carbon sub-step)")
+    return (f"diffuse_bulk_march_kernel<{form}, {nphase}, 6, {'true' if noflux else 'false'}, "
+            f"{rows}, {4 if mixed else 2}, 1>")
 
 
 def timed_run(dom, model, steps: int, warmup: int):
@@ -104,10 +144,45 @@ def timed_run(dom, model, steps: int, warmup: int):
         dom.step(1)
     dom.synchronize()
     wall = time.perf_counter() - t0
-    diff_ms, diff_n = dom.timing(0)
-    pf_ms, pf_n = dom.timing(1)
+    t = {"sub": dom.timing(0), "sweep": dom.timing(1), "march": dom.timing(2)}
     dom.timing_reset(False)
-    return wall, diff_ms, diff_n, pf_ms, pf_n
+    return wall, t
+
+
+# ---- host identification and CPU baselines -------------------------------------------------------
+def host_info() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc, "os_cpu_count": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+            "OMP_PLACES": os.environ.get("OMP_PLACES")}
+
+
+def _oracle_setup(m, p):
+    import oracle
+
+    d = m.desc()
+    od = oracle.OracleDomain(d.nx, d.ny, d.nz, d.depth, d.nsolute)
+    od.set_params(p)
+    grains = oracle.tessellation(d.nx, d.ny, d.nz, m.voronoi_seeds(), 0.0)
+    ic = [m.get_real(f"init_conc{i}") for i in range(3)]
+    if d.nsolute > 1:
+        ic += [m.get_real("mn_conc")] * 3
+    od.construct(grains, m.get_int("num_init_iterations"), ic)
+    od.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"), (0.8, -1.0), 1)
+    return od
 
 
 def cpu_baseline(workload: str, tmp: str, steps: int = 12):
@@ -120,14 +195,7 @@ def cpu_baseline(workload: str, tmp: str, steps: int = 12):
     m = Model(configs.write_inputs(os.path.join(tmp, "cpu"), workload, **over))
     p = m.update_parameter()
     d = m.desc()
-    od = oracle.OracleDomain(d.nx, d.ny, d.nz, d.depth, d.nsolute)
-    od.set_params(p)
-    grains = oracle.tessellation(d.nx, d.ny, d.nz, m.voronoi_seeds(), 0.0)
-    ic = [m.get_real(f"init_conc{i}") for i in range(3)]
-    if d.nsolute > 1:
-        ic += [m.get_real("mn_conc")] * 3
-    od.construct(grains, m.get_int("num_init_iterations"), ic)
-    od.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"), (0.8, -1.0), 1)
+    od = _oracle_setup(m, p)
     p2 = m.update_parameter()
     od.set_params(p2)
     t0 = time.perf_counter()
@@ -138,15 +206,61 @@ def cpu_baseline(workload: str, tmp: str, steps: int = 12):
             "cores": oracle.num_threads(), "kind": "port",
             "sample": f"{steps} full steps (1 sweep + {p2.n_sub[0]} carbon sub-steps"
                       f"{' + 1 Mn update' if d.nsolute > 1 else ''} each) of a 128^3 cut of the "
-                      f"workload (8 grains, 16 nuclei), FP64, {dt:.1f} s"}
+                      f"workload (8 grains, 16 nuclei), FP64, {dt:.1f} s",
+            "host": host_info()}
 
 
+def c0_baselines(tmp: str) -> dict:
+    """The reference's CPU-runnable case (2D 256^2 Fe-C, 12 grains + 20 nuclei, K = 6, FP64) in
+    full: 1000 steps on the oracle's OpenMP path, only the step loop timed (main.cpp:29-41), and
+    the same 1000 steps on the GPU in both forms."""
+    import oracle
+    from phasefield_amd import Domain, Model, configs
+
+    m = Model(configs.write_inputs(os.path.join(tmp, "c0"), "fec_2d_256"))
+    p = m.update_parameter()
+    d = m.desc()
+    steps = 1000
+    od = _oracle_setup(m, p)
+    s0 = od.get_state()
+    p2 = m.update_parameter()
+    od.set_params(p2)
+    log(f"cpu: fec_2d_256, {steps} steps on {oracle.num_threads()} threads")
+    t0 = time.perf_counter()
+    od.step(steps)
+    cpu_s = time.perf_counter() - t0
+    n = d.nx * d.ny * d.nz
+    out = {"cpu": {"value": n * steps / cpu_s / 1e9, "unit": "Gcell-updates/s",
+                   "cores": oracle.num_threads(), "kind": "port", "seconds": round(cpu_s, 2),
+                   "sample": f"fec_2d_256 in full: {steps} steps of 256x256 Fe-C (12 grains, 20 "
+                             f"nuclei, K = 6, FP64, {p2.n_sub[0]} sub-steps per step), step loop "
+                             f"only", "host": host_info()}}
+    for prec in ("fp64", "mixed"):
+        dd = m.desc()
+        dd.precision = 0 if prec == "fp64" else 1
+        g = Domain.from_desc(dd)
+        g.set_params(p2)
+        g.upload(s0["index"], s0["value"], s0["phase"], s0["conc"])
+        g.step(5)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.synchronize()
+        gs = time.perf_counter() - t0
+        g.close()
+        out[f"gpu_{prec}"] = {"value": n * steps / gs / 1e9, "unit": "Gcell-updates/s",
+                              "seconds": round(gs, 3)}
+    return out
+
+
+# ---- adjoint --------------------------------------------------------------------------------------
 def adjoint_section(tmp: str, steps: int | None, fd: list):
     """Discrete adjoint on the 128^3 Fe-C adjoint workload: targets from a forward run with
     mobility x1.1 and D_gamma x0.9, loss = sum over 20 sample times of (f_gamma - target)^2
     + 1e-3 ||c_final - c_target||^2, checkpoints every 20 steps.  Reports the reverse sweep's cost
-    against a plain forward run of the same steps, and central finite differences (relative step
-    1e-4) of the four gradients."""
+    against a plain forward run of the same steps, and central finite differences of the four
+    gradients at the given relative steps, with the number of cells whose final active-set maps
+    differ between the +h and -h runs."""
     import numpy as np
 
     from phasefield_amd import Model, configs, params_from_dict, params_to_dict
@@ -162,6 +276,7 @@ def adjoint_section(tmp: str, steps: int | None, fd: list):
     s0 = g.download()
     st = (s0["index"], s0["value"], s0["phase"], s0["conc"].copy())
     N = g.N
+    K = m.desc().depth
     samples = [ck * (q + 1) for q in range(nsteps // ck)][:20]
     base = params_to_dict(p0)
 
@@ -173,16 +288,18 @@ def adjoint_section(tmp: str, steps: int | None, fd: list):
             g.step(1)
             if k in samples:
                 fr.append(g.reduce()["phase_frac"][0])
-        c = g.download()["conc"]
+        s = g.download()
         if targets is not None:
-            loss = sum((a - b) ** 2 for a, b in zip(fr, targets)) + wc * float(np.sum((c - c_target) ** 2))
-        return loss, fr, c
+            loss = sum((a - b) ** 2 for a, b in zip(fr, targets)) + wc * float(
+                np.sum((s["conc"] - c_target) ** 2))
+        return loss, fr, s
 
     prob = AdjointProblem(g, base, st[0], st[1], st[2], nsteps, samples, [0.0] * len(samples), 1e-3,
                           None, ck)
     th0 = prob.theta0().numpy()
     # targets: mobility x1.1, D_gamma x0.9
-    _, f_t, c_t = forward_loss(th0 * np.array([1.1, 1.0, 0.9]), st[3])
+    _, f_t, s_t = forward_loss(th0 * np.array([1.1, 1.0, 0.9]), st[3])
+    c_t = s_t["conc"]
     prob.target_fraction = f_t
     prob.target_conc = c_t
     # plain forward of the same steps (no sampling), timed
@@ -196,6 +313,9 @@ def adjoint_section(tmp: str, steps: int | None, fd: list):
     r = prob.run(th0, st[3])
     out = {"workload": f"{wl}: 128^3 Fe-C, 32 grains + 40 nuclei, {nsteps} steps, "
                        f"{p0.n_sub[0]} sub-steps/step, checkpoints every {ck}, 20 samples",
+           "gradient": "exact derivative of the computed discrete map with the step's discrete "
+                       "decisions (active-set union, clamps, prunes, top-K cut, sort order) held "
+                       "fixed at the forward run's; dt and n_sub frozen",
            "loss": r["loss"], "forward_ms_per_step": round(fwd_s * 1e3 / nsteps, 4),
            "adjoint_forward_ms_per_step": round(r["forward_ms"] / nsteps, 4),
            "adjoint_reverse_ms_per_step": round(r["reverse_ms"] / nsteps, 4),
@@ -220,48 +340,75 @@ def adjoint_section(tmp: str, steps: int | None, fd: list):
                 if k < 3:
                     e = np.zeros(3)
                     e[k] = h * th0[k]
-                    lp = forward_loss(th0 + e, st[3], f_t, c_t)[0]
-                    lm = forward_loss(th0 - e, st[3], f_t, c_t)[0]
+                    lp, _, sp = forward_loss(th0 + e, st[3], f_t, c_t)
+                    lm, _, sm = forward_loss(th0 - e, st[3], f_t, c_t)
                     fdv = (lp - lm) / (2 * e[k])
                 else:
-                    lp = forward_loss(th0, st[3] + h * direction, f_t, c_t)[0]
-                    lm = forward_loss(th0, st[3] - h * direction, f_t, c_t)[0]
+                    lp, _, sp = forward_loss(th0, st[3] + h * direction, f_t, c_t)
+                    lm, _, sm = forward_loss(th0, st[3] - h * direction, f_t, c_t)
                     fdv = (lp - lm) / (2 * h)
-                rows.append({"fd": fdv, "rel_err": abs(grads[k] - fdv) / max(abs(fdv), 1e-300)})
+                differ = int(np.any(sp["index"].reshape(N, K) != sm["index"].reshape(N, K),
+                                    axis=1).sum())
+                rows.append({"fd": fdv, "rel_err": abs(grads[k] - fdv) / max(abs(fdv), 1e-300),
+                             "cells_with_different_maps": differ})
             out["fd_check"]["by_rel_step"][f"{h:g}"] = rows
-            log(f"adjoint: FD h={h:g} rel errors {[round(x['rel_err'], 6) for x in rows]}")
+            log(f"adjoint: FD h={h:g} rel errors {[round(x['rel_err'], 6) for x in rows]} "
+                f"map differences {[x['cells_with_different_maps'] for x in rows]}")
     g.close()
     return out
 
 
-def accuracy_run(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, nsolute, steps: int, every: int):
-    """Step both forms `steps` times from the same seeded state (each with its own per-step
-    parameter update) and report the mixed form's error against the FP64 form every `every`
-    steps: austenite fraction, max c per solute, solute-mass drift of each form."""
+# ---- accuracy -------------------------------------------------------------------------------------
+def accuracy_rows(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, nsolute, start: int, steps: int,
+                  every: int):
+    """Step both forms from step `start` to step `steps` (each with its own per-step parameter
+    update) and report the mixed form's error against the FP64 form every `every` steps:
+    austenite fraction, max c per solute, solute-mass drift of each form."""
     rows = []
     t0 = time.perf_counter()
-    for k in range(1, steps + 1):
+
+    def row(k):
+        smx, s64 = mixed.reduce(), g64.reduce()
+        fg64, fgmx = s64["phase_frac"][0], smx["phase_frac"][0]
+        r = {"step": k, "austenite_fraction": fgmx, "austenite_fraction_fp64": fg64,
+             "austenite_fraction_rel": abs(fgmx - fg64) / max(abs(fg64), 1e-300),
+             "max_c_rel": [abs(a - b) / abs(b) for a, b in
+                           zip(smx["max_conc"][:nsolute], s64["max_conc"][:nsolute])],
+             "mass_drift_rel_mixed": [(a - b) / b for a, b in
+                                      zip(smx["mass"][:nsolute], st0_mixed["mass"][:nsolute])],
+             "mass_drift_rel_fp64": [(a - b) / b for a, b in
+                                     zip(s64["mass"][:nsolute], st0["mass"][:nsolute])]}
+        log(f"accuracy step {k}: f_gamma rel {r['austenite_fraction_rel']:.3e} max c rel "
+            f"{[f'{x:.2e}' for x in r['max_c_rel']]} mass drift mixed "
+            f"{[f'{x:.2e}' for x in r['mass_drift_rel_mixed']]} ({time.perf_counter() - t0:.0f} s)")
+        return r
+
+    rows.append(row(start))
+    for k in range(start + 1, steps + 1):
         for dom, model in ((mixed, m_mixed), (g64, m_fp64)):
             dom.set_params(model.update_parameter())
             dom.step(1)
         if k % every == 0 or k == steps:
-            smx, s64 = mixed.reduce(), g64.reduce()
-            fg64, fgmx = s64["phase_frac"][0], smx["phase_frac"][0]
-            row = {"step": k,
-                   "austenite_fraction": fgmx, "austenite_fraction_fp64": fg64,
-                   "austenite_fraction_rel": abs(fgmx - fg64) / max(abs(fg64), 1e-300),
-                   "max_c_rel": [abs(a - b) / abs(b) for a, b in
-                                 zip(smx["max_conc"][:nsolute], s64["max_conc"][:nsolute])],
-                   "mass_drift_rel_mixed": [abs(a - b) / b for a, b in
-                                            zip(smx["mass"][:nsolute], st0_mixed["mass"][:nsolute])],
-                   "mass_drift_rel_fp64": [abs(a - b) / b for a, b in
-                                           zip(s64["mass"][:nsolute], st0["mass"][:nsolute])]}
-            rows.append(row)
-            log(f"accuracy step {k}: f_gamma rel {row['austenite_fraction_rel']:.3e} max c rel "
-                f"{[f'{x:.2e}' for x in row['max_c_rel']]} mass drift mixed "
-                f"{[f'{x:.2e}' for x in row['mass_drift_rel_mixed']]} "
-                f"({time.perf_counter() - t0:.0f} s)")
+            rows.append(row(k))
     return rows
+
+
+# ---- rocprofv3 counter files ----------------------------------------------------------------------
+def pmc_summary(directory: str) -> dict:
+    """Mean of each counter per kernel name over the dispatches in the rocprofv3 counter-collection
+    CSV files under `directory` (the per-dispatch values the profiles/ summaries quote)."""
+    acc = {}
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for rec in csv.DictReader(f):
+                key = (rec.get("Kernel_Name", ""), rec.get("Counter_Name", ""))
+                v = acc.setdefault(key, [0.0, 0])
+                v[0] += float(rec.get("Counter_Value", 0) or 0)
+                v[1] += 1
+    out = {}
+    for (kern, ctr), (tot, n) in sorted(acc.items()):
+        out.setdefault(kern, {})[ctr] = {"mean": tot / n, "dispatches": n}
+    return out
 
 
 def main() -> int:
@@ -273,19 +420,25 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--accuracy-steps", type=int, default=0,
-                    help="instead of timing: step mixed and FP64 this many steps and report the "
-                         "mixed form's error against FP64 along the way")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="after the timed steps, continue both forms to this step and report the "
+                         "mixed form's error against FP64 along the way (0: off)")
     ap.add_argument("--accuracy-every", type=int, default=100)
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=0)
-    ap.add_argument("--adjoint-fd", default="", metavar="H1,H2,..",
+    ap.add_argument("--adjoint-fd", default="1e-4,1e-8", metavar="H1,H2,..",
                     help="central finite differences of the adjoint gradients at these relative "
-                         "steps (8 forward runs each), e.g. 1e-4,1e-6")
+                         "steps (8 forward runs each); empty for none")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="only print the per-kernel mean of each counter in the rocprofv3 "
+                         "counter-collection files under DIR")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
+        return 0
     if args.calibrate_copy:
         import torch
 
@@ -301,7 +454,10 @@ def main() -> int:
         print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1 "
                                    "(multi-GPU halo exchange is not built)"}))
         return 2
-    from phasefield_amd import Domain, Model, PF_MIXED, configs
+    # the oracle's OpenMP runtime reads these when it starts (cpu_baseline)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from phasefield_amd import Domain, Model, configs
 
     tmp = tempfile.mkdtemp(prefix="pfbench_")
     fd_steps = [float(x) for x in args.adjoint_fd.split(",") if x.strip()]
@@ -327,49 +483,61 @@ def main() -> int:
     mixed.copy_from(g64)
     st0 = g64.reduce()
     st0_mixed = mixed.reduce()  # after rounding the FP64 state to FP32
-    if args.accuracy_steps:
-        rows = accuracy_run(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, d.nsolute,
-                            args.accuracy_steps, args.accuracy_every)
-        print(json.dumps({"workload": args.workload, "accuracy": rows,
-                          "mass_fp32_rounding_of_initial_state_rel": [
-                              abs(a - b) / b for a, b in zip(st0_mixed["mass"][: d.nsolute],
-                                                             st0["mass"][: d.nsolute])]}),
-              flush=True)
-        return 0
 
     res = {}
     traffic = measured_traffic(args.workload)
+    noflux = d.bc_z == 1
     for name, dom, model in (("mixed", mixed, m_mixed), ("fp64", g64, m_fp64)):
         if name == "fp64" and args.no_fp64:
             continue
+        is_mixed = name == "mixed"
+        form = "FormMixed" if is_mixed else "FormF64"
         log(f"{name}: {args.warmup} warmup + {args.steps} timed steps")
-        wall, dms, dn, pms, pn = timed_run(dom, model, args.steps, args.warmup)
+        wall, t = timed_run(dom, model, args.steps, args.warmup)
+        sweep_listed, diff_listed = dom.list_lengths()
+        rows, zc = dom.march_shape()
         st = dom.reduce()
-        launch_us = dms / dn * 1e3
-        bpc = alg_bytes_per_substep(name == "mixed", d.nphase)
-        achieved = N * bpc / (launch_us * 1e-6) / 1e9
-        measured = traffic.get("FormMixed" if name == "mixed" else "FormF64")
-        res[name] = {"wall": wall, "ms_per_step": wall / args.steps * 1e3,
-                     "value": N * args.steps / wall / 1e9, "diff_us": launch_us,
-                     "diff_launches": dn, "pf_us": pms / pn * 1e3, "stats": st,
-                     "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
-                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(achieved / HBM_PEAK_GBS, 4),
-                                  "traffic": measured,
-                                  "kernel": diffusion_kernel_name(name == "mixed", d.nphase, d.bc_z == 1),
-                                  "bytes_per_cell": bpc, "cells_per_launch": N,
-                                  "launch_us": round(launch_us, 2),
-                                  # the algorithmic model reads value_phase for every cell; bulk
-                                  # cells need none, so the measured bytes per launch are fewer
-                                  # and `frac` can pass 1.  The measured rate against the peak:
-                                  "traffic_gbs": (round(measured / (launch_us * 1e-6) / 1e9, 1)
-                                                  if measured else None),
-                                  "traffic_frac": (round(measured / (launch_us * 1e-6) / 1e9
-                                                         / HBM_PEAK_GBS, 4) if measured else None)}}
-        log(f"{name}: {res[name]['ms_per_step']:.2f} ms/step, diffusion {launch_us:.1f} us "
-            f"({achieved:.0f} GB/s alg), sweep {res[name]['pf_us']:.1f} us")
+        sub_us = t["sub"][0] / t["sub"][1] * 1e3
+        march_us = t["march"][0] / max(t["march"][1], 1) * 1e3
+        sweep_us = t["sweep"][0] / t["sweep"][1] * 1e3
+        bpc = march_bytes_per_cell(is_mixed)
+        achieved = N * bpc / (march_us * 1e-6) / 1e9
+        kname = march_kernel_name(is_mixed, d.nphase, noflux, rows)
+        measured = (traffic.get("march", {}).get(form)
+                    if traffic and not traffic.get("stale") and traffic.get("kernel", {}).get(form) == kname
+                    else None)
+        lbpc = listed_bytes_per_cell(is_mixed, d.nphase)
+        sub_bytes = N * bpc + diff_listed * lbpc
+        res[name] = {
+            "wall": wall, "ms_per_step": wall / args.steps * 1e3, "value": N * args.steps / wall / 1e9,
+            "stats": st,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": measured,
+                         "kernel": kname, "tile_rows": rows, "z_chunk": zc,
+                         "bytes_per_cell": bpc, "cells_per_launch": N,
+                         "launch_us": round(march_us, 2), "launches_timed": t["march"][1],
+                         "traffic_source": (f"{traffic.get('file')} (kernel sources "
+                                            f"{traffic.get('sources')}"
+                                            f"{', stale: not used' if traffic.get('stale') else ''})"
+                                            if traffic else None),
+                         "traffic_frac": (round(measured / (march_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+                                          if measured else None)},
+            "substep": {"us": round(sub_us, 2), "launches_timed": t["sub"][1],
+                        "listed_cells": diff_listed,
+                        "listed_fraction": round(diff_listed / N, 6),
+                        "bytes": sub_bytes, "listed_bytes_per_cell": lbpc,
+                        "achieved_gbs": round(sub_bytes / (sub_us * 1e-6) / 1e9, 1),
+                        "frac": round(sub_bytes / (sub_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)},
+            "sweep": {"us": round(sweep_us, 1), "listed_cells": sweep_listed,
+                      "listed_fraction": round(sweep_listed / N, 6)},
+        }
+        log(f"{name}: {res[name]['ms_per_step']:.2f} ms/step, march {march_us:.1f} us "
+            f"({achieved:.0f} GB/s), sub-step {sub_us:.1f} us ({res[name]['substep']['frac']:.3f} of "
+            f"peak, {diff_listed} listed), sweep {sweep_us:.0f} us ({sweep_listed} listed)")
 
     mx = res["mixed"]
+    n_sub = p0.n_sub[0]
     out = {
         "metric": METRIC, "value": round(mx["value"], 5), "unit": "Gcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -377,33 +545,52 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Voronoi + nucleation)",
         "config": {"workload": f"{args.workload}: {d.nx}^3 Fe-C{'-Mn' if d.nsolute > 1 else ''}, "
                                f"{wl['grains']} Voronoi ferrite grains + {wl['nuclei']} austenite "
-                               f"nuclei (r=4), {'no-flux z' if wl['bc_z'] == 'noflux' else 'periodic'}, "
+                               f"nuclei (r=4), {'no-flux z' if noflux else 'periodic'}, "
                                f"K={d.depth}, P={d.nphase}, "
-                               f"{p0.n_sub[0]} carbon sub-steps"
+                               f"{n_sub} carbon sub-steps"
                                f"{' + 1 Mn update' if d.nsolute > 1 else ''} per step",
                    "form": "mixed (FP32 + Kahan, FP64 prune/renormalise)",
                    "nx": d.nx, "ny": d.ny, "nz": d.nz},
         "roofline": mx["roofline"],
+        "substep": mx["substep"],
+        "sweep": mx["sweep"],
     }
+    for name in res:
+        b8 = model_8d_bytes_per_cell_step(name == "mixed", d.nphase, d.nsolute, n_sub)
+        gbs = N * b8 / (res[name]["ms_per_step"] * 1e-3) / 1e9
+        res[name]["model_8d"] = {"bytes_per_cell_step": b8, "achieved_gbs": round(gbs, 1),
+                                 "ratio_to_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    out["model_8d"] = mx["model_8d"]
     if "fp64" in res:
         f64 = res["fp64"]
-        s64, smx = f64["stats"], mx["stats"]
         out["fp64"] = {"value": round(f64["value"], 5), "ms_per_step": round(f64["ms_per_step"], 3),
-                       "dtype": "f64", "roofline": f64["roofline"]}
+                       "dtype": "f64", "roofline": f64["roofline"], "substep": f64["substep"],
+                       "sweep": f64["sweep"], "model_8d": f64["model_8d"]}
         out["mixed_over_fp64"] = round(mx["value"] / f64["value"], 4)
-        fg64, fgmx = s64["phase_frac"][0], smx["phase_frac"][0]
+        out["march_fp64_over_mixed"] = round(f64["roofline"]["launch_us"] / mx["roofline"]["launch_us"], 4)
+        done = args.warmup + args.steps
+        if args.accuracy_steps > done:
+            log(f"accuracy: both forms to step {args.accuracy_steps}")
+            rows = accuracy_rows(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, d.nsolute, done,
+                                 args.accuracy_steps, args.accuracy_every)
+        else:
+            rows = accuracy_rows(m_mixed, m_fp64, mixed, g64, st0, st0_mixed, d.nsolute, done,
+                                 done, 1)
+        last = rows[-1]
         out["error_vs_fp64"] = {
-            "after_steps": args.warmup + args.steps,
-            "austenite_fraction_rel": abs(fgmx - fg64) / max(abs(fg64), 1e-300),
-            "max_c_rel": [abs(a - b) / abs(b) for a, b in zip(smx["max_conc"][: d.nsolute],
-                                                            s64["max_conc"][: d.nsolute])],
-            "mass_drift_rel_mixed": [abs(a - b) / b for a, b in zip(smx["mass"][: d.nsolute],
-                                                                    st0_mixed["mass"][: d.nsolute])],
+            "after_steps": last["step"],
+            "austenite_fraction_rel": last["austenite_fraction_rel"],
+            "max_austenite_fraction_rel": max(r["austenite_fraction_rel"] for r in rows),
+            "max_c_rel": last["max_c_rel"],
+            "mass_drift_rel_mixed": last["mass_drift_rel_mixed"],
             "mass_fp32_rounding_of_initial_state_rel": [
                 abs(a - b) / b for a, b in zip(st0_mixed["mass"][: d.nsolute],
                                                st0["mass"][: d.nsolute])],
-            "mass_drift_rel_fp64": [abs(a - b) / b for a, b in zip(s64["mass"][: d.nsolute],
-                                                                   st0["mass"][: d.nsolute])],
+            "mass_drift_rel_fp64": last["mass_drift_rel_fp64"],
+            "trajectory": [{"step": r["step"], "f_gamma_rel": r["austenite_fraction_rel"],
+                            "max_c_rel": r["max_c_rel"],
+                            "mass_drift_mixed": r["mass_drift_rel_mixed"],
+                            "mass_drift_fp64": r["mass_drift_rel_fp64"]} for r in rows],
         }
     mixed.close()
     g64.close()
@@ -412,6 +599,7 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline (oracle, OpenMP)")
         out["cpu_baseline"] = cpu_baseline(args.workload, tmp)
+        out["configs0"] = c0_baselines(tmp)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -135,11 +135,19 @@ int pf_reduce(pf_handle *h, pf_stats *out);
 /* Wait for all queued work on the handle's stream. */
 int pf_synchronize(pf_handle *h);
 /* Device time in ms of the dominant kernel launches since the last reset, and how many launches
- * it covers (one HIP event pair on the handle's stream around each batch of back-to-back
- * launches); kind 0 = diffusion sub-step (every sub-step of a batch but its first, which also
- * classifies bulk cells), 1 = phase-field sweep. */
+ * it covers (HIP event pairs on the handle's stream); kind 0 = diffusion sub-step (one pair around
+ * each batch of back-to-back sub-steps, leaving out a batch's first sub-step, which also classifies
+ * bulk cells), 1 = phase-field sweep, 2 = the plane-marching bulk kernel alone (one pair around
+ * every eighth of those sub-steps' bulk launches). */
 int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches);
 int pf_kernel_timing_reset(pf_handle *h, int enable);
+/* Lengths of the per-sweep cell lists the step works through (synchronises): the cells the last
+ * phase-field sweep updated with the full formula (the rest were copied), and the cells the
+ * diffusion sub-steps update with the full formula (the rest take the bulk update). */
+int pf_list_lengths(pf_handle *h, int64_t *sweep_listed, int64_t *diffusion_listed);
+/* Tile rows and z-chunk of the plane-marching diffusion kernel on this handle (picked by timing the
+ * candidate shapes at the first sub-step); 0, 0 before that or when that kernel is not in use. */
+int pf_march_shape(const pf_handle *h, int *tile_rows, int *z_chunk);
 /* Size in bytes of one checkpoint of this handle.  save/restore copy the whole state, the mixed
  * form's sub-step counters and the grain-code tables to / from host memory of that size; restore
  * rejects a blob whose header names another domain shape, form or size. */

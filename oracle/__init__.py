@@ -69,6 +69,7 @@ def lib():
             "orc_mt19937_nth": (C.c_uint32, [C.c_uint32, _I]),
             "orc_splitmix64": (C.c_uint64, [C.c_uint64]),
             "orc_quantize_comp": (C.c_float, [C.c_float, C.c_float, C.c_uint32]),
+            "orc_quantize_comp_mean": (C.c_double, [C.c_float, C.c_float]),
             "orc_mix32": (C.c_uint32, [C.c_uint32]),
         }
         for name, (res, args) in sig.items():

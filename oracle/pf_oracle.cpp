@@ -95,9 +95,10 @@ using std::sqrt;
 
 // Mixed form: the Kahan compensation is kept as an 8-bit signed fixed-point multiple of 2^-7 ulp
 // of the stored value (comp = q * 2^(e - 31), value = m * 2^e with 0.5 <= |m| < 1, q saturating at
-// [-128, 127]), rounded stochastically: q = floor(x + u) for the scaled value x and a threshold u
-// in [0, 1) with 16 fraction bits hashed from the cell index and the sub-step (key).  This returns
-// the compensation that representation holds.
+// [-128, 127]), rounded stochastically without bias: X = the compensation in units of 2^-24 of that
+// quantum (nearest integer, saturated to [-128, 127] quanta), U = the high 24 bits of a hash of the
+// cell index and the sub-step (key), q = floor((X + U) / 2^24) in integers.  This returns the
+// compensation that representation holds.
 inline uint32_t mix32(uint32_t x) {  // xorshift-multiply integer hash ("lowbias32" form)
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -109,10 +110,20 @@ inline uint32_t mix32(uint32_t x) {  // xorshift-multiply integer hash ("lowbias
 inline float quantize_comp(float comp, float value, uint32_t key) {
   int e = 0;
   (void)std::frexp(value, &e);
-  const float x = std::ldexp(comp, 31 - e);
-  float q = std::floor(x + float(key >> 16) * 0x1p-16f);
-  q = std::fmin(std::fmax(q, -128.f), 127.f);
-  return std::ldexp(q, e - 31);
+  float s = std::ldexp(comp, 31 + 24 - e);
+  s = std::fmin(std::fmax(s, -2147483648.f), 2130706432.f);
+  const int64_t X = int64_t(std::nearbyint(s));
+  const int64_t q = (X + int64_t(key >> 8)) >> 24;  // floor: arithmetic shift of a signed value
+  return std::ldexp(float(q), e - 31);
+}
+// the mean of quantize_comp over all 2^24 thresholds (exact in int64)
+inline double quantize_comp_mean(float comp, float value) {
+  int64_t sum = 0;
+  int e = 0;
+  (void)std::frexp(value, &e);
+  for (uint32_t u = 0; u < (1u << 24); ++u)
+    sum += int64_t(std::ldexp(quantize_comp(comp, value, u << 8), 31 - e));
+  return std::ldexp(double(sum), e - 31 - 24);
 }
 template <class T> inline T quantize_comp(T comp, T, uint32_t) { return comp; }
 
@@ -144,6 +155,7 @@ inline uint64_t noise_bits(uint64_t seed, uint64_t counter) {
 
 struct DomainBase {
   virtual ~DomainBase() {}
+  int nsolute = 1;  // solutes the domain carries
   virtual void set_params(const orc_params *p) = 0;
   virtual void set_state(const int32_t *, const double *, const double *, const double *) = 0;
   virtual void get_state(int32_t *, double *, double *, double *, double *, double *) = 0;
@@ -177,6 +189,7 @@ template <class F> struct Domain : DomainBase {
 
   Domain(int x, int y, int z, int k, int ns, int bc)
       : nx(x), ny(y), nz(z), K(k), NS(ns), bcz(bc), N(long(x) * y * z) {
+    nsolute = ns;
     idx1.assign(N * K, 0);
     idx2.assign(N * K, 0);
     val1.assign(N * K, S(0.0));
@@ -432,9 +445,8 @@ template <class F> struct Domain : DomainBase {
             if (F::kahan) {
               const S y = S(del) - comp[s][c];
               const S t = conc[s][c] + y;
-              // cells 2j and 2j + 1 take the high and low halves of key ^ (j * 0x9E3779B9)
-              const uint32_t hj = key ^ (uint32_t(c >> 1) * 0x9E3779B9u);
-              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t, (c & 1) ? hj << 16 : hj);
+              // the cell's rounding hash: key ^ (c * 0x9E3779B9)
+              comp[s][c] = quantize_comp((t - conc[s][c]) - y, t, key ^ (uint32_t(c) * 0x9E3779B9u));
               conc[s][c] = t;
             } else {
               conc[s][c] += S(del);
@@ -746,7 +758,7 @@ void orc_step(void *h, int n_steps) {
   DomainBase *d = dom(h);
   for (int it = 0; it < n_steps; ++it) {
     d->pf_sweep();
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < d->nsolute; ++s)
       if (d->p.n_sub[s] > 0) d->diffuse(s, d->p.n_sub[s], d->p.dt_d[s] / (d->p.dx * d->p.dx));
   }
 }
@@ -867,6 +879,7 @@ uint32_t orc_mt19937_nth(uint32_t seed, int n) {
   return v;
 }
 uint64_t orc_splitmix64(uint64_t x) { return splitmix64(x); }
+double orc_quantize_comp_mean(float comp, float value) { return quantize_comp_mean(comp, value); }
 float orc_quantize_comp(float comp, float value, uint32_t key) {
   return quantize_comp(comp, value, key);
 }

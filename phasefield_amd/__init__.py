@@ -200,6 +200,18 @@ class Domain:
         _check(lib().pf_kernel_timing(self._h, kind, C.byref(ms), C.byref(n)), self._h, "timing")
         return ms.value, n.value
 
+    def list_lengths(self) -> tuple[int, int]:
+        """(cells the last sweep listed, cells the diffusion sub-steps list)."""
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib().pf_list_lengths(self._h, C.byref(a), C.byref(b)), self._h, "list lengths")
+        return a.value, b.value
+
+    def march_shape(self) -> tuple[int, int]:
+        """(tile rows, z-chunk) of the plane-marching diffusion kernel; (0, 0) before its first use."""
+        a, b = C.c_int(), C.c_int()
+        _check(lib().pf_march_shape(self._h, C.byref(a), C.byref(b)), self._h, "march shape")
+        return a.value, b.value
+
     # --- checkpoints -------------------------------------------------------------------------
     def checkpoint(self) -> bytes:
         n = C.c_size_t()

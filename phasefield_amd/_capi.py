@@ -89,6 +89,8 @@ SIGNATURES = {
     "pf_dump_wait": (_I, [_H]),
     "pf_kernel_timing": (_I, [_H, _I, C.POINTER(_D), C.POINTER(C.c_int64)]),
     "pf_kernel_timing_reset": (_I, [_H, _I]),
+    "pf_list_lengths": (_I, [_H, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "pf_march_shape": (_I, [_H, C.POINTER(_I), C.POINTER(_I)]),
     "pf_checkpoint_size": (_I, [_H, C.POINTER(C.c_size_t)]),
     "pf_checkpoint_save": (_I, [_H, C.c_void_p]),
     "pf_checkpoint_restore": (_I, [_H, C.c_void_p]),

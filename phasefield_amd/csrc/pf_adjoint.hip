@@ -562,6 +562,9 @@ struct StepData {
 
 template <int K, int P, int NF, bool NOF> struct Adj {
   static int run(pf_handle *h, const pf_adjoint_spec *sp, pf_adjoint_result *out) {
+    // the run starts from the state as it is, partitioned with the current tables (which the run
+    // keeps fixed)
+    handle_state_changed(h);
     const HandleView v = handle_view(h);
     const long N = v.N;
     const int KI = v.KI;
@@ -621,11 +624,16 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       return handle_fail(h, PF_ERR_OOM,
                          std::string("adjoint buffers: ") + hipGetErrorString(ar.err));
 
+    // first error of the asynchronous copies and clears below (reported by launch_ok)
+    hipError_t async_err = hipSuccess;
+    auto async_ok = [&](hipError_t e) {
+      if (async_err == hipSuccess) async_err = e;
+    };
     auto copy_state = [&](const DevState &dst, const DevState &src) {
-      hipMemcpyAsync(dst.id, src.id, size_t(N) * KI * 2, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(dst.phi, src.phi, n8 * K, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(dst.pf, src.pf, n8 * P, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(dst.c, src.c, n8, hipMemcpyDeviceToDevice, st);
+      async_ok(hipMemcpyAsync(dst.id, src.id, size_t(N) * KI * 2, hipMemcpyDeviceToDevice, st));
+      async_ok(hipMemcpyAsync(dst.phi, src.phi, n8 * K, hipMemcpyDeviceToDevice, st));
+      async_ok(hipMemcpyAsync(dst.pf, src.pf, n8 * P, hipMemcpyDeviceToDevice, st));
+      async_ok(hipMemcpyAsync(dst.c, src.c, n8, hipMemcpyDeviceToDevice, st));
     };
     auto on_handle = [&]() {
       const HandleView w = handle_view(h);
@@ -647,14 +655,14 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       return -1;
     };
     auto launch_ok = [&]() -> int {
-      const hipError_t e = hipGetLastError();
+      const hipError_t e = async_err != hipSuccess ? async_err : hipGetLastError();
       return e == hipSuccess ? PF_OK
                              : handle_fail(h, PF_ERR_HIP, std::string("adjoint: ") +
                                                               hipGetErrorString(e));
     };
 
     // ---- forward with checkpoints ----
-    if (target) hipMemcpyAsync(target, sp->target_conc, n8, hipMemcpyHostToDevice, st);
+    if (target) async_ok(hipMemcpyAsync(target, sp->target_conc, n8, hipMemcpyHostToDevice, st));
     const auto t0 = std::chrono::steady_clock::now();
     double loss = 0.0;
     std::vector<double> frac(sp->n_samples, 0.0);
@@ -673,20 +681,20 @@ template <int K, int P, int NF, bool NOF> struct Adj {
     }
     if (rc != PF_OK) return rc;
     save(fin);
-    hipMemsetAsync(lam_c, 0, n8, st);
-    hipMemsetAsync(lam_phi, 0, n8 * K, st);
-    hipMemsetAsync(lam_pf, 0, n8 * P, st);
-    hipMemsetAsync(accD, 0, size_t(gb) * 8, st);
-    hipMemsetAsync(accMS, 0, size_t(gs) * 16, st);
+    async_ok(hipMemsetAsync(lam_c, 0, n8, st));
+    async_ok(hipMemsetAsync(lam_phi, 0, n8 * K, st));
+    async_ok(hipMemsetAsync(lam_pf, 0, n8 * P, st));
+    async_ok(hipMemsetAsync(accD, 0, size_t(gb) * 8, st));
+    async_ok(hipMemsetAsync(accMS, 0, size_t(gs) * 16, st));
     if (target && sp->weight_conc != 0.0) {
       hipLaunchKernelGGL(adj_seed_conc_kernel, dim3(gb), dim3(256), 0, st, fin.c, target,
                          sp->weight_conc, lam_c, part, N);
       std::vector<double> hp(gb);
-      hipMemcpyAsync(hp.data(), part, size_t(gb) * 8, hipMemcpyDeviceToHost, st);
-      if (hipStreamSynchronize(st) != hipSuccess) return launch_ok();
+      async_ok(hipMemcpyAsync(hp.data(), part, size_t(gb) * 8, hipMemcpyDeviceToHost, st));
+      if (hipStreamSynchronize(st) != hipSuccess || async_err != hipSuccess) return launch_ok();
       for (double x : hp) loss += x;
     }
-    if (hipStreamSynchronize(st) != hipSuccess) return launch_ok();
+    if (hipStreamSynchronize(st) != hipSuccess || async_err != hipSuccess) return launch_ok();
     const auto t1 = std::chrono::steady_clock::now();
 
     // ---- reverse ----
@@ -696,15 +704,15 @@ template <int K, int P, int NF, bool NOF> struct Adj {
       int r = pf_phase_field_sweep(h);
       if (r != PF_OK) return r;
       const DevState cur = on_handle();
-      hipMemcpyAsync(sd.pf_new, cur.pf, n8 * P, hipMemcpyDeviceToDevice, st);
+      async_ok(hipMemcpyAsync(sd.pf_new, cur.pf, n8 * P, hipMemcpyDeviceToDevice, st));
       (void)launch_classify<double, P, NF, NOF>(st, sd.pf_new, unit_tmp, sd.flag, v.nx, v.ny,
                                                  v.nz, N, v.bulk_mask, sd.count);
-      hipMemcpyAsync(sd.csub, cur.c, n8, hipMemcpyDeviceToDevice, st);
+      async_ok(hipMemcpyAsync(sd.csub, cur.c, n8, hipMemcpyDeviceToDevice, st));
       for (int s = 0; s < nsub && r == PF_OK; ++s)
         r = handle_diffuse_io(h, sd.csub + size_t(N) * s, sd.csub + size_t(N) * (s + 1));
       if (r != PF_OK) return r;
       // the step's output carbon, the next step's input
-      hipMemcpyAsync(cur.c, sd.csub + size_t(N) * nsub, n8, hipMemcpyDeviceToDevice, st);
+      async_ok(hipMemcpyAsync(cur.c, sd.csub + size_t(N) * nsub, n8, hipMemcpyDeviceToDevice, st));
       return launch_ok();
     };
     // reverse step k from slot q (sub-step data) and its input state
@@ -725,7 +733,8 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         hipLaunchKernelGGL(adj_list_flag_kernel, dim3(gb), dim3(256), 0, st, sd.flag, N);
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
                            ablock);
-        hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl));
+        hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl),
+                           static_cast<int *>(nullptr));
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
                            ablock, alist);
       }
@@ -765,8 +774,8 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         }
         std::swap(lam_c, lam_c2);
       }
-      hipMemsetAsync(lam_phi2, 0, n8 * K, st);
-      hipMemsetAsync(lam_pf2, 0, n8 * P, st);
+      async_ok(hipMemsetAsync(lam_phi2, 0, n8 * K, st));
+      async_ok(hipMemsetAsync(lam_pf2, 0, n8 * P, st));
       hipLaunchKernelGGL((adj_sweep_kernel<K, P, NF, NOF>), dim3(gs), dim3(kAdjBS), 0, st, in.id,
                          in.phi, in.pf, in.c, lam_phi, lam_pf, lam_phi2, lam_pf2, lam_c, v2_id,
                          v2_lam, v2_n, accMS, cs, v.nx, v.ny, v.nz, N);
@@ -799,10 +808,12 @@ template <int K, int P, int NF, bool NOF> struct Adj {
     }
     if (rc == PF_OK) {
       std::vector<double> hD(gb), hMS(size_t(gs) * 2);
-      hipMemcpyAsync(hD.data(), accD, hD.size() * 8, hipMemcpyDeviceToHost, st);
-      hipMemcpyAsync(hMS.data(), accMS, hMS.size() * 8, hipMemcpyDeviceToHost, st);
-      if (out->d_conc0) hipMemcpyAsync(out->d_conc0, lam_c, n8, hipMemcpyDeviceToHost, st);
-      const hipError_t e = hipStreamSynchronize(st);
+      async_ok(hipMemcpyAsync(hD.data(), accD, hD.size() * 8, hipMemcpyDeviceToHost, st));
+      async_ok(hipMemcpyAsync(hMS.data(), accMS, hMS.size() * 8, hipMemcpyDeviceToHost, st));
+      if (out->d_conc0)
+        async_ok(hipMemcpyAsync(out->d_conc0, lam_c, n8, hipMemcpyDeviceToHost, st));
+      hipError_t e = hipStreamSynchronize(st);
+      if (e == hipSuccess) e = async_err;
       if (e != hipSuccess) rc = handle_fail(h, PF_ERR_HIP, hipGetErrorString(e));
       const auto t2 = std::chrono::steady_clock::now();
       double dD = 0.0, dM = 0.0, dS = 0.0;

@@ -55,11 +55,11 @@ struct pf_handle {
   std::string err;
   // timing of the dominant launches
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];
-  std::vector<int> ev_count[2];  // launches covered by each event pair
-  size_t ev_used[2] = {0, 0};
-  double t_ms[2] = {0, 0};
-  int64_t t_n[2] = {0, 0};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  std::vector<int> ev_count[3];  // launches covered by each event pair
+  size_t ev_used[3] = {0, 0, 0};
+  double t_ms[3] = {0, 0, 0};
+  int64_t t_n[3] = {0, 0, 0};
   int diff_zchunk_override = 0;
   int diff_variant = 0;  // 0: bulk-aware column kernel, 1: LDS-tiled full-formula kernel
   uint8_t *flag = nullptr;   // per-cell bulk phase (0 = needs the full formula)
@@ -74,12 +74,23 @@ struct pf_handle {
   bool diff_march = true;    // ... in its plane-marching form
   int march_by = 0;          // its tile rows and z-chunk, picked by timing at first use
   int march_zc = 0;
+  int *list_len = nullptr;   // [0]: cells the last sweep listed, [1]: cells the diffusion lists
+  bool time_march = false;   // time the next plane-marching launch (timing kind 2)
   int *glist = nullptr;      // indices of the listed cells, storage order
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
   bool sweep_split = true;   // copy cells classified and written apart from the listed sweep
   uint8_t *sflag = nullptr;  // per cell: 1 = copied by the last sweep
   bool sweep_hist = false;   // sflag describes the last sweep of the current buffers
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
+  // The reference stores the partitioned concentrations (value_conc1) and reads them until they are
+  // recomputed, so after a parameter change it still reads partitions of the previous tables: in
+  // the sweep's driving force, and in the first sub-step for the cells the sweep copied.  The
+  // tables the stored state was last partitioned with, per solute:
+  double prev_pk[PF_MAX_SOLUTE][3][3] = {}, prev_pc[PF_MAX_SOLUTE][3][3] = {};
+  bool have_prev = false;
+  // next sub-step of solute s mixes two tables: 1 = the cells the last sweep copied keep the
+  // previous ones, 2 = every cell does (no sweep since the change)
+  int retable[PF_MAX_SOLUTE] = {0, 0};
   // asynchronous VTI dump: device fields -> pinned host copy (dump stream) -> writer thread
   void *dump_dev = nullptr;      // korn [N] int32, conc [N] float
   void *dump_host = nullptr;     // pinned copy of dump_dev
@@ -124,8 +135,9 @@ template <class A> PFConsts<A> make_pf_consts(const pf_handle *h) {
       c.ent[i][j] = A(in ? p.entropy[i - 1][j - 1] : 0.0);
       c.eqc[i][j] = A(in ? p.eq_conc[i - 1][j - 1] : 0.0);
       c.slope[i][j] = A(in ? p.eq_slope[i - 1][j - 1] : 1.0);
-      c.pk[i][j] = A(in ? p.part_k[0][i - 1][j - 1] : 0.0);
-      c.pc[i][j] = A(in ? p.part_c[0][i - 1][j - 1] : 0.0);
+      // local_conc of Driving_force: the stored partition, i.e. of the tables last used
+      c.pk[i][j] = A(in ? (h->have_prev ? h->prev_pk[0] : p.part_k[0])[i - 1][j - 1] : 0.0);
+      c.pc[i][j] = A(in ? (h->have_prev ? h->prev_pc[0] : p.part_c[0])[i - 1][j - 1] : 0.0);
       c.pkS[i][j] = A(in ? p.part_k[1][i - 1][j - 1] : 0.0);
       c.pcS[i][j] = A(in ? p.part_c[1][i - 1][j - 1] : 0.0);
     }
@@ -172,8 +184,39 @@ void update_bulk_mask(pf_handle *h) {
   h->flag_valid = false;
 }
 
+// the stored state now counts as partitioned with the current tables of solute s (all when s < 0)
+void tables_current(pf_handle *h, int s = -1) {
+  for (int q = 0; q < PF_MAX_SOLUTE; ++q)
+    if (s < 0 || s == q) {
+      std::memcpy(h->prev_pk[q], h->p.part_k[q], sizeof(h->prev_pk[q]));
+      std::memcpy(h->prev_pc[q], h->p.part_c[q], sizeof(h->prev_pc[q]));
+      h->retable[q] = 0;
+    }
+  if (s < 0) h->have_prev = h->have_params;
+}
+// the state was replaced: partitioned with the current tables, or with the first ones set
+void tables_state_set(pf_handle *h) {
+  tables_current(h);
+  h->have_prev = h->have_params;
+}
+// whether solute s's previous and current tables differ in the form's arithmetic type
+bool tables_differ(const pf_handle *h, int s) {
+  if (!h->have_prev) return false;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double a = h->prev_pk[s][i][j], b = h->p.part_k[s][i][j];
+      const double c = h->prev_pc[s][i][j], d = h->p.part_c[s][i][j];
+      if (is_mixed(h) ? (float(a) != float(b) || float(c) != float(d)) : (a != b || c != d))
+        return true;
+    }
+  return false;
+}
+
 // ---- launch helpers -----------------------------------------------------------------------------
 inline unsigned blocks_for(long n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+// the plane-marching launch is timed on every kMarchTimingEvery-th sub-step of a batch (timing
+// kind 2): a sample of its launches, with few events between the kernels of the step
+constexpr int kMarchTimingEvery = 8;
 // workgroups of the full-formula list build (kListChunk * 256 cells each)
 inline unsigned list_blocks(long n) { return blocks_for(n, 256 * kListChunk); }
 
@@ -211,7 +254,8 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
   const T *phi1 = static_cast<const T *>(h->phi[a]);
   T *phi2 = static_cast<T *>(h->phi[b]);
   const C *c0 = static_cast<const C *>(h->conc[0][h->ccur[0]]);
-  if (!ISO && h->sweep_split) {
+  // the split form records which cells it copied, which a change of partition tables needs
+  if (!ISO && (h->sweep_split || h->retable[0] || h->retable[1])) {
     // copy cells first (written, or skipped when the destination already holds them), then the
     // full update over the list of the others
     const unsigned nb = blocks_for(h->N), nl = list_blocks(h->N);
@@ -220,7 +264,8 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
                        h->p.cutoff, h->d.nx, h->d.ny, h->d.nz, h->N);
     hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
                        static_cast<const unsigned *>(nullptr), h->N, h->gblock);
-    hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock, int(nl));
+    hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock, int(nl),
+                       h->list_len);
     hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
                        static_cast<const unsigned *>(nullptr), h->N, h->gblock, h->glist);
     // grid-stride over the list: enough workgroups to fill every CU at the LDS-limited occupancy
@@ -416,7 +461,7 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, keep,
                            h->N, h->gblock);
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock,
-                           int(nl));
+                           int(nl), h->list_len + 1);
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag,
                            keep, h->N, h->gblock, h->glist);
       }
@@ -447,6 +492,8 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
         go(std::integral_constant<int, 4>(), split);
     };
     if (march) {
+      // kind 2: this launch alone, on sampled sub-steps once the shape is picked
+      Timer t(h, 2, h->time_march && h->march_by != 0 ? 1 : 0);
       const hipError_t e = launch_march<F, P, NF, NOF>(h, dc, cin, cout);
       if (e != hipSuccess) return e;
     } else if (h->diff_split && h->diff_vec && h->d.nx % V == 0 &&
@@ -475,8 +522,27 @@ template <class F, int P, int NF, bool NOF>
 hipError_t launch_diffuse(pf_handle *h, int s, const DiffConsts<typename F::A> &dc) {
   using C = typename F::C;
   const int a = h->ccur[s], b = 1 - a;
-  const hipError_t e = launch_diffuse_io<F, P, NF, NOF>(
-      h, dc, static_cast<const C *>(h->conc[s][a]), static_cast<C *>(h->conc[s][b]));
+  const C *cin = static_cast<const C *>(h->conc[s][a]);
+  C *cout = static_cast<C *>(h->conc[s][b]);
+  hipError_t e;
+  if (h->retable[s]) {
+    // the first sub-step after a sweep that changed the partition tables: full formula for every
+    // cell, copied cells partitioned with the previous tables (their stored partitions)
+    DiffConsts<typename F::A> prev = dc;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        prev.pk[i][j] = typename F::A(h->prev_pk[s][i][j]);
+        prev.pc[i][j] = typename F::A(h->prev_pc[s][i][j]);
+      }
+    hipLaunchKernelGGL((diffuse_retable_kernel<F, P, NF, NOF>), dim3(blocks_for(h->N)), dim3(256),
+                       0, h->stream, cin, cout, static_cast<const typename F::T *>(h->pfrac),
+                       h->retable[s] == 1 ? h->sflag : static_cast<const uint8_t *>(nullptr), dc,
+                       prev, h->d.nx, h->d.ny, h->d.nz);
+    e = hipGetLastError();
+    tables_current(h, s);
+  } else {
+    e = launch_diffuse_io<F, P, NF, NOF>(h, dc, cin, cout);
+  }
   h->ccur[s] = b;
   return e;
 }
@@ -492,6 +558,10 @@ template <class F>
 int run_diffuse(pf_handle *h, int s, int n, double factor) {
   auto dc = make_diff_consts<typename F::A>(h->p, s, factor);
   if (n <= 0) return PF_OK;
+  if (h->have_state && !h->have_prev) tables_state_set(h);
+  // a change of tables with no sweep since: the first sub-step reads every cell's partition of
+  // the previous tables; every cell is partitioned with the current tables after this batch
+  if (!h->retable[s] && tables_differ(h, s)) h->retable[s] = 2;
   // the first launch classifies bulk cells when the flags are stale; keep it out of the timing
   dc.key = comp_step_key(h->csub[s]++, s);
   hipError_t e = h->d.nphase == 2 ? dispatch_diffuse_p<F, 2>(h, s, dc)
@@ -500,13 +570,21 @@ int run_diffuse(pf_handle *h, int s, int n, double factor) {
   Timer t(h, 0, n - 1);
   for (int it = 1; it < n; ++it) {
     dc.key = comp_step_key(h->csub[s]++, s);
+    h->time_march = h->timing && it % kMarchTimingEvery == 0;
     e = h->d.nphase == 2 ? dispatch_diffuse_p<F, 2>(h, s, dc) : dispatch_diffuse_p<F, 3>(h, s, dc);
+    h->time_march = false;
     if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("diffuse: ") + hipGetErrorString(e));
   }
+  tables_current(h, s);
   return PF_OK;
 }
 
 template <class F> int run_sweep(pf_handle *h) {
+  if (h->have_state && !h->have_prev) tables_state_set(h);
+  // cells the sweep updates are partitioned with the current tables from here on, the copied ones
+  // keep the tables they were last partitioned with: the next sub-step of a solute whose tables
+  // changed mixes the two
+  for (int s = 0; s < h->d.nsolute; ++s) h->retable[s] = tables_differ(h, s) ? 1 : 0;
   const auto cs = make_pf_consts<typename F::A>(h);
   Timer t(h, 1);
   hipError_t e = dispatch_sweep<F, false>(h, cs);
@@ -538,10 +616,11 @@ void free_all(pf_handle *h) {
   if (h->unit) hipFree(h->unit);
   if (h->bulk_count) hipFree(h->bulk_count);
   if (h->glist) hipFree(h->glist);
+  if (h->list_len) hipFree(h->list_len);
   if (h->sflag) hipFree(h->sflag);
   if (h->gblock) hipFree(h->gblock);
   if (h->red) hipFree(h->red);
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < 3; ++k)
     for (auto &pr : h->ev[k]) {
       hipEventDestroy(pr.first);
       hipEventDestroy(pr.second);
@@ -635,6 +714,9 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if ((rc = chk(hipMalloc(&h->bulk_count, 4 * sizeof(unsigned)), "alloc bulk counts")) != PF_OK)
     return rc;
   if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->list_len, 2 * sizeof(int)), "alloc list lengths")) != PF_OK) return rc;
+  if ((rc = chk(hipMemsetAsync(h->list_len, 0, 2 * sizeof(int), h->stream), "clear")) != PF_OK)
+    return rc;
   if ((rc = chk(hipMalloc(&h->sflag, N), "alloc sweep copy flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->gblock, (list_blocks(h->N) + 1) * sizeof(int)),
                 "alloc list offsets")) != PF_OK)
@@ -818,6 +900,7 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
   h->have_state = true;
   h->flag_valid = false;
   h->sweep_hist = false;
+  tables_state_set(h);
   return PF_OK;
 }
 
@@ -1022,14 +1105,16 @@ int pf_kernel_timing_reset(pf_handle *h, int enable) {
   hipSetDevice(h->d.device);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->timing = enable != 0;
-  h->ev_used[0] = h->ev_used[1] = 0;
-  h->t_ms[0] = h->t_ms[1] = 0.0;
-  h->t_n[0] = h->t_n[1] = 0;
+  for (int k = 0; k < 3; ++k) {
+    h->ev_used[k] = 0;
+    h->t_ms[k] = 0.0;
+    h->t_n[k] = 0;
+  }
   return PF_OK;
 }
 
 int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches) {
-  if (!h || kind < 0 || kind > 1) return PF_ERR_INVALID;
+  if (!h || kind < 0 || kind > 2) return PF_ERR_INVALID;
   hipSetDevice(h->d.device);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (size_t i = 0; i < h->ev_used[kind]; ++i) {
@@ -1041,6 +1126,23 @@ int pf_kernel_timing(pf_handle *h, int kind, double *total_ms, int64_t *launches
   h->ev_used[kind] = 0;
   if (total_ms) *total_ms = h->t_ms[kind];
   if (launches) *launches = h->t_n[kind];
+  return PF_OK;
+}
+
+int pf_march_shape(const pf_handle *h, int *tile_rows, int *z_chunk) {
+  if (!h) return PF_ERR_INVALID;
+  handle_march_shape(h, tile_rows, z_chunk);
+  return PF_OK;
+}
+
+int pf_list_lengths(pf_handle *h, int64_t *sweep_listed, int64_t *diffusion_listed) {
+  if (!h) return PF_ERR_INVALID;
+  hipSetDevice(h->d.device);
+  int v[2] = {0, 0};
+  HIPCHK(h, hipMemcpyAsync(v, h->list_len, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (sweep_listed) *sweep_listed = v[0];
+  if (diffusion_listed) *diffusion_listed = v[1];
   return PF_OK;
 }
 
@@ -1151,6 +1253,7 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
   h->have_state = true;
   h->flag_valid = false;
   h->sweep_hist = false;
+  tables_state_set(h);
   return PF_OK;
 }
 
@@ -1216,6 +1319,7 @@ int pf_setup_construct(pf_handle *h, const double *seeds, int G, double reductio
   h->have_state = true;
   h->flag_valid = false;
   h->sweep_hist = false;
+  tables_state_set(h);
   return PF_OK;
 }
 
@@ -1324,6 +1428,7 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
   h->code_to_id[1] = table;
   h->flag_valid = false;
   h->sweep_hist = false;
+  tables_state_set(h);
   return PF_OK;
 }
 
@@ -1341,6 +1446,7 @@ int pf_setup_solute_noise(pf_handle *h, int solute, double mean, double stddev, 
                      stddev, key, h->N, static_cast<double *>(h->conc[solute][h->ccur[solute]]));
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  tables_state_set(h);
   return PF_OK;
 }
 
@@ -1392,6 +1498,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
   dst->have_state = true;
   dst->flag_valid = false;
   dst->sweep_hist = false;
+  tables_state_set(dst);
   return PF_OK;
 }
 
@@ -1485,6 +1592,7 @@ int pf_gpu_diffusion_solver(const double *hdiffusivity, const double *hpartition
   h->csub[0] = 0;
   h->have_state = true;
   h->flag_valid = false;
+  tables_state_set(h);  // the reference partitions the uploaded state first (gpu_solver.cpp:210)
   h->NF = 6;  // Diff3D always sums six faces
   rc = diffuse_any(h, 0, num_loop, dt_dxdx);
   h->NF = 2 * h->dims;
@@ -1559,6 +1667,7 @@ int handle_diffuse_io(pf_handle *h, const double *cin, double *cout) {
 void handle_state_changed(pf_handle *h) {
   h->flag_valid = false;
   h->sweep_hist = false;
+  tables_state_set(h);
 }
 PFConsts<double> handle_pf_consts(const pf_handle *h) { return make_pf_consts<double>(h); }
 DiffConsts<double> handle_diff_consts(const pf_handle *h, int solute, double factor) {

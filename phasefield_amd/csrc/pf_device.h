@@ -42,12 +42,15 @@ template <class C> __host__ __device__ inline C conc_hi(C c) { return c; }
 // |comp| <= ulp/2 after a Kahan update, so |q| <= 64 with 2x headroom; q saturates at [-128, 127].
 // The effective concentration is c - comp.
 //
-// q is rounded stochastically: down or up to a neighbouring multiple with probability given by the
-// distance, from a 16-bit threshold u hashed from the cell index and the sub-step.  Rounding to
-// nearest would drop every increment below half a quantum (an increment too small to move c goes
-// into comp and is rounded away again, sub-step after sub-step), and those losses do not cancel
-// between neighbours of different exponents; with the unbiased rounding they average out.  x + u is
-// exact for |x| < 256 (u has 16 fraction bits), so a comp already on the grid stays unchanged.
+// q is rounded stochastically and without bias: with X = comp in units of 2^-24 quanta (rounded to
+// the nearest integer, saturated to [-128, 127] quanta) and a threshold U = the high 24 bits of a
+// per-cell hash of the sub-step, q = floor((X + U) / 2^24), computed in integers.  Over uniformly
+// distributed U the mean of q is X / 2^24 exactly.  (Thresholds with fewer fraction bits than the
+// compensation round down more often than up: with 16-bit thresholds the stored compensation was
+// low by up to 2^-16 quanta in every cell with a nonzero increment, a systematic mass gain that
+// grew as the diffusion front reached more cells.)  Rounding to nearest would drop every increment
+// below half a quantum, and those losses do not cancel between neighbours of different exponents.
+// A compensation already on the grid is kept by every threshold; threshold 0 rounds down.
 using CompCode = int8_t;
 constexpr int kCompShift = 31;
 __host__ __device__ inline int comp_exp(float c) {
@@ -68,25 +71,23 @@ __host__ __device__ inline uint32_t mix32(uint32_t x) {
   return x;
 }
 // key of one sub-step of solute s: k counts the sub-steps applied to that solute since the state
-// was set; the rounding thresholds of cells 2j and 2j + 1 are then the high and low halves of
-// key ^ (j * 0x9E3779B9) (comp_hash).  Over the sub-steps of one cell the thresholds are as
-// uniform as the keys, which is all the unbiased rounding needs.
+// was set; the rounding hash of cell i is key ^ (i * 0x9E3779B9) (comp_hash).  Over the sub-steps
+// of one cell its thresholds are as uniform as the keys' high 24 bits, which is all the unbiased
+// rounding needs.
 __host__ __device__ inline uint32_t comp_step_key(uint32_t k, int s) {
   return mix32(k * 2u + uint32_t(s));
 }
-// rounding hash of cell i (its threshold in the high 16 bits)
-constexpr uint32_t kCompPairMul = 0x9E3779B9u;
+constexpr uint32_t kCompCellMul = 0x9E3779B9u;
 __host__ __device__ inline uint32_t comp_hash(uint32_t i, uint32_t key) {
-  const uint32_t h = key ^ ((i >> 1) * kCompPairMul);
-  return (i & 1u) ? h << 16 : h;
+  return key ^ (i * kCompCellMul);
 }
 // the code of compensation comp of value c with rounding hash h (h = 0 rounds down: exact for a
 // compensation already on the grid, as in uploads)
 __host__ __device__ inline CompCode comp_encode(float comp, float c, uint32_t h) {
-  const float x = ldexpf(comp, kCompShift - comp_exp(c));
-  float v = floorf(x + float(h >> 16) * 0x1p-16f);
-  v = fminf(fmaxf(v, -128.f), 127.f);
-  return CompCode(int(v));
+  float s = ldexpf(comp, kCompShift + 24 - comp_exp(c));
+  s = fminf(fmaxf(s, -2147483648.f), 2130706432.f);  // [-128, 127] quanta in units of 2^-24
+  const int X = int(rintf(s));
+  return CompCode((X + int(h >> 8)) >> 24);  // floor((X + U) / 2^24), never beyond [-128, 127]
 }
 // effective stored concentration of cell i of a buffer of n cells (mixed: value - compensation)
 template <class F> __device__ inline double conc_effective(const typename F::C *buf, long i, long n);

@@ -533,10 +533,10 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
                                                 uint32_t key) {
   VecA<float, 4> o;
   VecCode<4> qo;
-  // i is a multiple of 4: cells i, i+1 share one hash, i+2, i+3 the next (comp_hash)
-  const uint32_t j = uint32_t(i) >> 1;
-  const uint32_t h0 = key ^ (j * kCompPairMul), h1 = key ^ ((j + 1u) * kCompPairMul);
-  const uint32_t h[4] = {h0, h0 << 16, h1, h1 << 16};
+  // rounding hashes of cells i..i+3 (comp_hash) from one product
+  const uint32_t base = uint32_t(i) * kCompCellMul;
+  const uint32_t h[4] = {key ^ base, key ^ (base + kCompCellMul), key ^ (base + 2u * kCompCellMul),
+                         key ^ (base + 3u * kCompCellMul)};
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const float comp = comp_decode(qi.q[v], c.v[v]);
@@ -924,6 +924,51 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
   }
 }
 
+// The reference keeps each cell's partitioned concentrations (value_conc1) from the last time it
+// computed them: the last diffusion sub-step for the cells a sweep copies, the sweep itself for the
+// cells it updates (solver.cpp:207-216, 300-302; phase_field.cpp:273-283).  When the partition tables
+// change between steps (a temperature path), the first sub-step after the sweep therefore reads
+// partitions of the previous step's tables in the copied cells and of the current tables in the
+// others.  This kernel reproduces that sub-step: the full formula for every cell, one thread per
+// cell, each of the seven cells partitioned with the tables its flag names.
+template <class F, int P, int NF, bool NOFLUX>
+__global__ __launch_bounds__(256) void diffuse_retable_kernel(
+    const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
+    const typename F::T *__restrict__ pfrac, const uint8_t *__restrict__ copied,
+    const DiffConsts<typename F::A> dc, const DiffConsts<typename F::A> dc_prev, const int nx,
+    const int ny, const int nz) {
+  using A = typename F::A;
+  const long plane = long(nx) * ny, ncell = plane * nz;
+  const long i = long(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= ncell) return;
+  int x, y, z;
+  cell_coords(i, nx, ny, x, y, z);
+  long nbi[6];
+  nbi[0] = i - x + wrap(x - 1, nx);
+  nbi[1] = i - x + wrap(x + 1, nx);
+  nbi[2] = i + long(nx) * (wrap(y - 1, ny) - y);
+  nbi[3] = i + long(nx) * (wrap(y + 1, ny) - y);
+  const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
+  const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
+  nbi[4] = i + plane * (zm - z);
+  nbi[5] = i + plane * (zp - z);
+  StencilLoad<F, P, NF> s;
+  stencil_load<F, P, NF>(cin, pfrac, i, ncell, nbi, s);
+  CellPF<A> me;
+  partition_cell<F, P>(s.me, copied[i] ? dc_prev : dc, me);
+  A del = A(0);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    CellPF<A> nb;
+    partition_cell<F, P>(s.nb[f], copied[nbi[f]] ? dc_prev : dc, nb);
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+      del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
+  }
+  del *= dc.r;
+  conc_store<F>(cout, i, ncell, s.me.c, del, s.q, dc.key);
+}
+
 // ---- list of the cells that need the full formula, in storage order -----------------------------
 __device__ inline bool listed_cell(int f, const unsigned *count, int keep) {
   return count ? f != keep : f == 0;
@@ -966,7 +1011,8 @@ __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__res
 }
 
 // exclusive prefix sum of bcount[0..nb) in place; bcount[nb] = total.  One workgroup of 1024.
-__global__ __launch_bounds__(1024) void general_scan_kernel(int *__restrict__ bcount, int nb) {
+__global__ __launch_bounds__(1024) void general_scan_kernel(int *__restrict__ bcount, int nb,
+                                                             int *__restrict__ total_out) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
   const int per = (nb + 1023) / 1024;
@@ -987,7 +1033,10 @@ __global__ __launch_bounds__(1024) void general_scan_kernel(int *__restrict__ bc
     bcount[k] = run;
     run += c;
   }
-  if (t == 1023) bcount[nb] = part[1023];
+  if (t == 1023) {
+    bcount[nb] = part[1023];
+    if (total_out) *total_out = part[1023];  // the list length, kept for diagnostics
+  }
 }
 
 __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__restrict__ flag,
@@ -1042,6 +1091,9 @@ PF_INST_CLASSIFY(float, 3)
   template __global__ void diffuse_general_kernel<F, P, NF, NOF>(                                 \
       const F::C *, F::C *, const F::T *, const int *, const int *, const DiffConsts<F::A>, int,  \
       int, int);                                                                                \
+  template __global__ void diffuse_retable_kernel<F, P, NF, NOF>(                                 \
+      const F::C *, F::C *, const F::T *, const uint8_t *, const DiffConsts<F::A>,              \
+      const DiffConsts<F::A>, int, int, int);                                                   \
   PF_INST_BULK(F, P, NF, NOF, 4, false)                                                         \
   PF_INST_BULK(F, P, NF, NOF, 8, false)                                                         \
   PF_INST_BULK(F, P, NF, NOF, 16, false)                                                        \

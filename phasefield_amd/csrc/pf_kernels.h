@@ -90,13 +90,27 @@ __global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
                                        const DiffConsts<typename F::A> dc, const int nx,
                                        const int ny, const int nz);
 
+// The first sub-step after a sweep whose partition tables differ from those the stored
+// concentrations were last partitioned with: the full formula for every cell, each cell's partition
+// taken with the previous tables (dc_prev) if the sweep copied it (copied[cell] == 1), else with the
+// current ones.  Diffusivities, factor and rounding key come from dc.
+template <class F, int P, int NF, bool NOFLUX>
+__global__ void diffuse_retable_kernel(const typename F::C *__restrict__ cin,
+                                       typename F::C *__restrict__ cout,
+                                       const typename F::T *__restrict__ pfrac,
+                                       const uint8_t *__restrict__ copied,
+                                       const DiffConsts<typename F::A> dc,
+                                       const DiffConsts<typename F::A> dc_prev, const int nx,
+                                       const int ny, const int nz);
+
 // List of the cells a full-formula pass must cover, in storage order: flag == 0, or, with a
 // non-null `count`, flag != bulk_phase(count) (the cells the plane-marching kernel gets wrong).
 constexpr int kListChunk = 16;  // 256-cell chunks per block of the full-formula list build
 __global__ void general_count_kernel(const uint8_t *__restrict__ flag,
                                      const unsigned *__restrict__ count, long ncell,
                                      int *__restrict__ bcount);
-__global__ void general_scan_kernel(int *__restrict__ bcount, int nb);
+__global__ void general_scan_kernel(int *__restrict__ bcount, int nb,
+                                    int *__restrict__ total_out);
 __global__ void general_scatter_kernel(const uint8_t *__restrict__ flag,
                                        const unsigned *__restrict__ count, long ncell,
                                        const int *__restrict__ boff, int *__restrict__ list);

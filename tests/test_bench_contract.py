@@ -19,16 +19,34 @@ def test_more_than_one_gpu_is_refused():
     assert "error" in json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def test_algorithmic_bytes_per_sub_step():
-    # read P phase fractions + the stored concentration, write the concentration
-    assert bench.alg_bytes_per_substep(False, 2) == 2 * 8 + 8 + 8
-    assert bench.alg_bytes_per_substep(True, 2) == 2 * 4 + 5 + 5  # FP32 value + one-byte code
-    assert bench.alg_bytes_per_substep(True, 3) == 3 * 4 + 5 + 5
+def test_byte_models():
+    # plane-marching kernel: the stored concentration read and written once per cell
+    assert bench.march_bytes_per_cell(False) == 8 + 8
+    assert bench.march_bytes_per_cell(True) == 5 + 5  # FP32 value + one-byte code each way
+    # a listed cell: its own and six neighbours' concentration + value_phase, one write
+    assert bench.listed_bytes_per_cell(False, 2) == 7 * (8 + 16) + 8
+    assert bench.listed_bytes_per_cell(True, 2) == 7 * (5 + 8) + 5
+    # the scope document's model per step at 200 sub-steps, K = 4, P = 2, Mn once per step
+    assert bench.model_8d_bytes_per_cell_step(False, 2, 2, 200) == 6560
+    assert bench.model_8d_bytes_per_cell_step(True, 2, 2, 200) == 4904
 
 
-def test_measured_traffic_is_read_from_profiles():
+def test_measured_traffic_names_its_sources():
     t = bench.measured_traffic("fecmn_3d_512")
-    assert set(t) >= {"FormF64", "FormMixed"}
-    # at least the compulsory bytes the marching kernel moves per cell (16 / 10 B at 512^3)
-    assert t["FormF64"] >= 16 * 512 ** 3 and t["FormMixed"] >= 10 * 512 ** 3
-    assert t["FormF64"] > t["FormMixed"]
+    assert t, "profiles/ holds no traffic line for the default workload"
+    assert t["stale"] == (t.get("sources") != bench.kernel_source_hash())
+    if not t["stale"]:
+        m = t["march"]
+        # at least the compulsory bytes the marching kernel moves per cell (16 / 10 B at 512^3)
+        assert m["FormF64"] >= 16 * 512 ** 3 and m["FormMixed"] >= 10 * 512 ** 3
+
+
+def test_pmc_summary_reads_counter_files(tmp_path):
+    d = tmp_path / "host" / "123"
+    d.mkdir(parents=True)
+    (d / "run_counter_collection.csv").write_text(
+        "Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n"
+        "1,k_a,FETCH_SIZE,10\n2,k_a,FETCH_SIZE,30\n3,k_b,FETCH_SIZE,5\n")
+    out = bench.pmc_summary(str(tmp_path))
+    assert out["k_a"]["FETCH_SIZE"] == {"mean": 20.0, "dispatches": 2}
+    assert out["k_b"]["FETCH_SIZE"]["mean"] == 5.0

@@ -111,13 +111,16 @@ def test_fecmn_1024_workload_small_bitwise(tmp_path, form):
     assert_same(gs, os_, 4, "fecmn_3d_1024 small")
 
 
-def test_temperature_path_with_arrhenius_bitwise(tmp_path):
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+def test_temperature_path_with_arrhenius_bitwise(tmp_path, form):
     """Temperature 1013 K -> 995 K over the first 6 ms, then towards 1030 K, with Q != 0 for the
     mobility (140 kJ/mol) and both diffusivities (80 / 140 kJ/mol): every step takes the host
     model's Update_parameter (phase_field.cpp:232-242, 307-315), which matches the oracle's
     restatement bitwise, and the device and the oracle step with those values; the sub-step count
-    changes along the path.  Fields and maps agree bitwise after every step."""
-    banner("temperature path + Arrhenius")
+    and the partition tables change along the path, so the driving force and the first sub-step
+    read partitions of the previous step's tables where the reference's stored value_conc1 still
+    holds them.  Fields and maps agree bitwise after every step."""
+    banner(f"temperature path + Arrhenius, form {form}")
     R, T0 = 8.314, 1013.0
     QM, QA, QG = 140e3, 80e3, 140e3
     wl = dict(nx=48, ny=32, nz=24, grains=5, nuclei=4,
@@ -137,9 +140,12 @@ def test_temperature_path_with_arrhenius_bitwise(tmp_path):
     ent = np.array([[1, 2, 0.0, -0.2e-12], [1, 3, 0.0, 0.0], [2, 3, 0.0, 0.0]])
     text = configs.thermo_text().split("#PARA wt%\n")[1]
     para = np.array([[float(x) for x in ln.split()] for ln in text.strip().splitlines()])
-    g = Domain(48, 32, 24, depth=4)
+    # the state is partitioned with the tables in force when it is set (Construct's parameters)
+    g = Domain(48, 32, 24, depth=4, precision="mixed" if form == oracle.ORC_MIXED else "fp64")
+    g.set_params(p0)
     g.upload(*st)
-    o = oracle.OracleDomain(48, 32, 24, 4, 1)
+    o = oracle.OracleDomain(48, 32, 24, 4, 1, form)
+    o.set_params(p0)
     o.set_state(*st)
     n_subs, temps = set(), []
     for k in range(10):
@@ -158,7 +164,9 @@ def test_temperature_path_with_arrhenius_bitwise(tmp_path):
         g.step(1)
         o.set_params(p)
         o.step(1)
-        assert_same(g.download(), o.get_state(), 4, f"T path step {k + 1} (T = {T:.2f} K)")
+        gs, os_ = g.download(), o.get_state()
+        assert_same(gs, os_, 4, f"T path step {k + 1} (T = {T:.2f} K)")
+        assert np.array_equal(gs["conc_comp"], os_["conc_comp"])
     assert len(n_subs) >= 3, n_subs
     assert min(temps) < 1000.0 and temps[-1] > min(temps)
     g.close()
@@ -174,13 +182,14 @@ def _two_grain_uniform(N, K):
     return ids.reshape(-1), val.reshape(-1), phase.reshape(-1)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_all_phi_below_cutoff_gives_the_reference_nan(base_params_any, precision):
+@pytest.mark.parametrize("precision,nphase", [("fp64", 3), ("mixed", 3), ("fp64", 2)])
+def test_all_phi_below_cutoff_gives_the_reference_nan(base_params_any, precision, nphase):
     """Both driving forces negative: in every cell both grains fall below the cutoff, so the sum is
     0 and the reference's renormalisation `*= 1./sum` (solver.cpp:292-296) writes NaN.  The device
     reproduces that state exactly (NaN in the same places, empty maps), and the reductions refuse it
-    with PF_ERR_NONFINITE."""
-    banner(f"sum = 0 -> NaN ({precision})")
+    with PF_ERR_NONFINITE.  A domain carrying two phases stores no phase-3 fraction (the
+    reference's 0 * inf there is NaN too); its two stored phases are compared."""
+    banner(f"sum = 0 -> NaN ({precision}, P = {nphase})")
     nx, ny, nz, K = 16, 12, 8, 4
     N = nx * ny * nz
     q = oracle.as_orc_params(base_params_any)
@@ -193,7 +202,7 @@ def test_all_phi_below_cutoff_gives_the_reference_nan(base_params_any, precision
     ids, val, phase = _two_grain_uniform(N, K)
     form = oracle.ORC_MIXED if precision == "mixed" else oracle.ORC_FP64
     gp = PfParams.from_buffer_copy(q)
-    g = Domain(nx, ny, nz, depth=K, precision=precision)
+    g = Domain(nx, ny, nz, depth=K, nphase=nphase, precision=precision)
     g.set_params(gp)
     g.upload(ids, val, phase, np.full(N, 1.0))
     g.sweep()
@@ -204,8 +213,10 @@ def test_all_phi_below_cutoff_gives_the_reference_nan(base_params_any, precision
     gs, os_ = g.download(), o.get_state()
     assert np.all(os_["index"] == 0) and np.all(np.isnan(os_["value"]))
     assert np.array_equal(gs["index"], os_["index"])
-    for k in ("value", "phase"):
-        assert np.array_equal(gs[k], os_[k], equal_nan=True), k
+    assert np.array_equal(gs["value"], os_["value"], equal_nan=True)
+    ph_g, ph_o = gs["phase"].reshape(N, 3), os_["phase"].reshape(N, 3)
+    assert np.array_equal(ph_g[:, :nphase], ph_o[:, :nphase], equal_nan=True)
+    assert np.all(np.isnan(ph_g[:, :nphase]))
     with pytest.raises(PhaseFieldError) as e:
         g.reduce()
     assert e.value.code == PF_ERR_NONFINITE

@@ -338,13 +338,25 @@ def test_noflux_equals_mirrored_periodic(params3d, rng):
 
 def test_compensation_rounding_is_unbiased_and_exact():
     """The mixed form's one-byte Kahan compensation (multiples of 2^-7 ulp of the value, saturating
-    at -128..127) is rounded stochastically with thresholds from hashed keys: averaged over keys it
-    equals the unrounded compensation (rounding to nearest would return 0 for 0.3 quanta every
-    time), a compensation already on the grid is kept by every key, key 0 rounds down, and the
-    range saturates."""
+    at -128..127) is rounded stochastically with thresholds from hashed keys.
+    - Over all 2^24 thresholds the mean of the stored compensation equals the compensation exactly
+      (exhaustive, in integers), also for compensations with bits far below a quantum; thresholds
+      with 16 fraction bits would give floor(f 2^16)/2^16 of a quantum for the fraction f instead.
+    - Over hashed keys the mean is the compensation within sampling noise (rounding to nearest
+      would return 0 for 0.3 quanta every time).
+    - A compensation already on the grid is kept by every key, key 0 rounds down, and the range
+      saturates."""
     L = oracle.lib()
     q = lambda comp, value, key: L.orc_quantize_comp(comp, value, key)
     keys = [L.orc_mix32(i) for i in range(1 << 15)]
+    for value, ms in ((0.02, (0x4CCCCD + 1, -0x2AAAAB, 3)),
+                      (1.7, (0x4CCCCD, 5 * (1 << 24) + 0x34568, -(41 << 24) - (7 << 6)))):
+        e = int(np.frexp(np.float32(value))[1])
+        quantum = float(np.ldexp(1.0, e - 31))
+        for m in ms:
+            comp = float(np.float32(m * 2.0 ** -24 * quantum))
+            assert comp == m * 2.0 ** -24 * quantum  # exactly representable
+            assert L.orc_quantize_comp_mean(comp, value) == comp, (value, m)
     for value in (0.5, 0.75, 0.02, 1.7, 3.0e-3):
         e = int(np.frexp(np.float32(value))[1])
         quantum = float(np.ldexp(1.0, e - 31))
