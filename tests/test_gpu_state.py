@@ -224,9 +224,9 @@ def test_all_phi_below_cutoff_gives_the_reference_nan(base_params_any, precision
 
 
 def test_unstable_diffusion_reaches_nonfinite(base_params_any, rng):
-    """A diffusion sub-step 50x beyond the explicit stability limit grows the concentration's
-    shortest wave by a factor ~600 per sub-step until it overflows; pf_reduce then reports
-    PF_ERR_NONFINITE instead of returning statistics."""
+    """A diffusion sub-step 1000x the workload's (r = D dt/dx^2 ~ 67 in ferrite) grows the
+    concentration's shortest wave by a factor ~800 per sub-step, past the FP64 range within about
+    110 of the 200 sub-steps; pf_reduce then reports PF_ERR_NONFINITE instead of statistics."""
     banner("unstable diffusion -> PF_ERR_NONFINITE")
     nx, ny, nz, K = 24, 16, 12, 4
     ids, val, phase, conc = helpers.random_state(rng, nx, ny, nz, K)
@@ -236,7 +236,7 @@ def test_unstable_diffusion_reaches_nonfinite(base_params_any, rng):
     r = g.reduce()
     assert r["n_nonfinite"] == 0
     p = base_params_any
-    g.diffuse(0, 200, p.dt_d[0] * 50)
+    g.diffuse(0, 200, p.dt_d[0] * 1000)
     with pytest.raises(PhaseFieldError) as e:
         g.reduce()
     assert e.value.code == PF_ERR_NONFINITE
