@@ -277,6 +277,19 @@ int pf_gpu_diffusion_solver(const double *hdiffusivity, const double *hpartition
                             double *htotal_conc, int x, int y, int z, double dt, double dx);
 /* Frees the shim's domain: GPU_Cleanup, gpu_solver.cpp:223-239. */
 void pf_gpu_cleanup(void);
+/* No counterpart in the reference: the allocation-free form of pf_gpu_diffusion_solver, for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same sub-step count and arithmetic (bitwise the same total_conc), but
+ * value_phase [N][3] and total_conc [N] are device pointers, the sub-steps are queued on `stream`
+ * (a hipStream_t; NULL for the null stream) and the call returns without waiting.  It allocates
+ * nothing: its second concentration buffer is a device array of the library with room for 2^21
+ * cells (16 MiB on every GPU the library is loaded on, kept until the process ends); a larger
+ * domain returns PF_ERR_RANGE.  Calls whose work overlaps on one GPU share that buffer and race;
+ * calls ordered on one stream do not.  Errors: pf_last_error(NULL). */
+int pf_gpu_diffusion_solver_noalloc(const double *hdiffusivity, const double *hpartition_k,
+                                    const double *hpartition_c, const double *dvalue_phase,
+                                    double *dtotal_conc, int x, int y, int z, double dt,
+                                    double dx, void *stream);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,9 @@ SIGNATURES = {
     "pf_gpu_diffusion_solver": (_I, [C.POINTER(_D), C.POINTER(_D), C.POINTER(_D), C.POINTER(_D),
                                      C.POINTER(_D), _I, _I, _I, _D, _D]),
     "pf_gpu_cleanup": (None, []),
+    "pf_gpu_diffusion_solver_noalloc": (_I, [C.POINTER(_D), C.POINTER(_D), C.POINTER(_D),
+                                             C.c_void_p, C.c_void_p, _I, _I, _I, _D, _D,
+                                             C.c_void_p]),
 }
 
 _lib = None

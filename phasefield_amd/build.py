@@ -38,7 +38,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
               "-I", os.path.join(ROOT, "include")]
 
 HIP_SOURCES = ["pf_sweep_f64.hip", "pf_sweep_mixed.hip", "pf_sweep_list.hip", "pf_diffuse.hip",
-               "pf_misc.hip", "pf_api.hip", "pf_adjoint.hip"]
+               "pf_misc.hip", "pf_api.hip", "pf_adjoint.hip", "pf_noalloc.hip"]
 HOST_SOURCES = ["host_model.cpp", "gpu_solver_compat.cpp"]
 
 

@@ -1639,6 +1639,7 @@ HandleView handle_view(pf_handle *h) {
   return v;
 }
 int handle_fail(pf_handle *h, int code, const std::string &msg) { return fail(h, code, msg); }
+int set_global_error(int code, const std::string &msg) { return fail(nullptr, code, msg); }
 void handle_march_shape(const pf_handle *h, int *rows, int *zchunk) {
   const bool used = h->diff_split && h->diff_vec && h->diff_march && h->diff_variant == 0;
   *rows = used ? h->march_by : 0;

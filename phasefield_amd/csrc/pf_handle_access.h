@@ -22,6 +22,8 @@ struct HandleView {
 };
 
 HandleView handle_view(pf_handle *h);
+// record an error that belongs to no handle (pf_last_error(NULL) returns it) and return `code`
+int set_global_error(int code, const std::string &msg);
 // record an error message on the handle and return `code`
 int handle_fail(pf_handle *h, int code, const std::string &msg);
 // the device state was replaced behind the step functions' back (drops cached bulk flags)
