@@ -161,10 +161,10 @@ def host_info() -> dict:
     except OSError:
         pass
     try:
-        nproc = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        nproc = os.cpu_count()
-    return {"cpu_model": model, "nproc": nproc, "os_cpu_count": os.cpu_count(),
+        affinity = None
+    return {"cpu_model": model, "cpus": os.cpu_count(), "main_thread_affinity_cpus": affinity,
             "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
             "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
             "OMP_PLACES": os.environ.get("OMP_PLACES")}

@@ -157,6 +157,28 @@ template <int K> __device__ inline void load_ids(const uint16_t *base, long cell
 #pragma unroll
   for (int s = 0; s < K; ++s) g[s] = (w[s >> 1] >> ((s & 1) * 16)) & 0xffffu;
 }
+// the K slot values of a cell's row with vector loads (rows are 8-byte aligned for every K and
+// type here; 16-byte pieces where the row size allows)
+template <class T, int K> __device__ inline void load_row(const T *base, long cell, T (&v)[K]) {
+  constexpr int B = int(sizeof(T)) * K;  // row bytes: 16, 24, 32 or 48
+  if constexpr (B % 16 == 0) {
+    using V = uint4;
+    const V *p = reinterpret_cast<const V *>(base + cell * K);
+#pragma unroll
+    for (int q = 0; q < B / 16; ++q) {
+      const V w = p[q];
+      __builtin_memcpy(reinterpret_cast<char *>(v) + 16 * q, &w, 16);
+    }
+  } else {
+    using V = uint2;
+    const V *p = reinterpret_cast<const V *>(base + cell * K);
+#pragma unroll
+    for (int q = 0; q < B / 8; ++q) {
+      const V w = p[q];
+      __builtin_memcpy(reinterpret_cast<char *>(v) + 8 * q, &w, 8);
+    }
+  }
+}
 template <int K> __device__ inline void store_ids(uint16_t *base, long cell, const uint32_t (&g)[K]) {
   using V = typename IdRowT<K>::V;
   V v;

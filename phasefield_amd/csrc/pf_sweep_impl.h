@@ -57,34 +57,6 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   cell_coords(cell, nx, ny, x, y, z);
   const double cutoff = cs.cutoff;
 
-  for (int e = 0; e < kStencilMax; ++e) {
-    sLg[e][t] = 0;
-    sV2[e][t] = T(0);
-  }
-  uint32_t g0[K];
-  T v0[K];
-  load_ids<K>(id1, cell, g0);
-#pragma unroll
-  for (int s = 0; s < K; ++s) v0[s] = phi1[cell * K + s];
-  int n = 0;
-#pragma unroll
-  for (int s = 0; s < K; ++s) {
-    if (double(v0[s]) > cutoff) {
-      ++n;
-      sLg[s][t] = uint16_t(g0[s]);
-    }
-  }
-
-  // the entry values of the own slots: an entry's own value is the slot value at its position
-  // when that slot is above the cutoff (positions >= K hold neighbour-only entries: 0)
-  auto v1_of = [&](int e) -> T {
-    T r = T(0);
-#pragma unroll
-    for (int s = 0; s < K; ++s)
-      if (e == s && double(v0[s]) > cutoff) r = v0[s];
-    return r;
-  };
-
   long nb[6];
   {
     const long plane = long(nx) * ny;
@@ -104,18 +76,58 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     nb[4] = x + long(nx) * y + plane * zm;
     nb[5] = x + long(nx) * y + plane * zp;
   }
+  // Every load of the cell is issued here, before any of them is used: its own and its
+  // neighbours' whole id and phi rows (vectors, unconditionally; only the slots before a row's first
+  // zero id are used), its value_phase and concentration.  One round of memory latency per cell
+  // instead of one per face.
+  uint32_t g0[K], gn[NF][K];
+  T v0[K], vn[NF][K];
+  load_ids<K>(id1, cell, g0);
+  load_row<T, K>(phi1, cell, v0);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    load_ids<K>(id1, nb[f], gn[f]);
+    load_row<T, K>(phi1, nb[f], vn[f]);
+  }
+  A ph[P], c = A(0);
+  if (!ISO) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) ph[j] = A(pfrac[cell * P + j]);
+    c = A(conc_hi(conc0[cell]));
+  }
+
+  for (int e = 0; e < kStencilMax; ++e) {
+    sLg[e][t] = 0;
+    sV2[e][t] = T(0);
+  }
+  int n = 0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (double(v0[s]) > cutoff) {
+      ++n;
+      sLg[s][t] = uint16_t(g0[s]);
+    }
+  }
+
+  // the entry values of the own slots: an entry's own value is the slot value at its position
+  // when that slot is above the cutoff (positions >= K hold neighbour-only entries: 0)
+  auto v1_of = [&](int e) -> T {
+    T r = T(0);
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+      if (e == s && double(v0[s]) > cutoff) r = v0[s];
+    return r;
+  };
 
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    uint32_t gn[K];
-    load_ids<K>(id1, nb[f], gn);
     bool live = true;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-      const uint32_t g = gn[s];
+      const uint32_t g = gn[f][s];
       live = live && (g != 0);
       if (live) {
-        const T v = phi1[nb[f] * K + s];
+        const T v = vn[f][s];
         if (!ISO || double(v) > cutoff) {
           int kk = 0;
           for (; kk < kStencilMax; ++kk)
@@ -148,10 +160,6 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
 #pragma unroll
     for (int b = 0; b < 4; ++b) dG[a][b] = A(0);
   if (!ISO) {
-    A ph[P];
-#pragma unroll
-    for (int j = 0; j < P; ++j) ph[j] = A(pfrac[cell * P + j]);
-    const A c = A(conc_hi(conc0[cell]));
     A lc[4];
     lc[0] = A(0);
 #pragma unroll
@@ -253,8 +261,18 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
   };
   if (LIST) {
+    // Workgroup b runs on XCD b % 8: each XCD walks a contiguous eighth of the (storage-order)
+    // list, so the neighbour rows of a workgroup's cells are more often in its own XCD's L2.
+    // (Fewer than 8 workgroups: one plain grid-stride walk, so that every range has a walker.)
     const int n = *count;
-    for (int k = blockIdx.x * kSweepBS + t; k < n; k += gridDim.x * kSweepBS) body(list[k]);
+    if (gridDim.x < 8) {
+      for (int k = blockIdx.x * kSweepBS + t; k < n; k += gridDim.x * kSweepBS) body(list[k]);
+    } else {
+      const int xcd = int(blockIdx.x & 7u), rank = int(blockIdx.x >> 3);
+      const int nb_x = (int(gridDim.x) - xcd + 7) / 8;  // workgroups on this XCD (>= 1)
+      const int lo = int((long(n) * xcd) / 8), hi = int((long(n) * (xcd + 1)) / 8);
+      for (int k = lo + rank * kSweepBS + t; k < hi; k += nb_x * kSweepBS) body(list[k]);
+    }
   } else {
     const long cell = long(blockIdx.x) * kSweepBS + t;
     if (cell < ncell) body(cell);
