@@ -654,13 +654,13 @@ __device__ inline void lds_barrier() {
 // R consecutive rows per wavefront, and walks its z-chunk one plane at a time.  The -z / own / +z
 // runs of each lane stay in registers; the y neighbours come from the lane's own other rows, from
 // the tile's rows staged in LDS (double-buffered, one barrier per plane) or, at the tile's edge
-// rows, from global memory; the x neighbours at the ends of a lane's run are loaded from the cache
-// lines the adjacent lanes read.  Only the tile's edge rows and the z-chunk's two end planes are
-// read from HBM a second time, which is what the batched kernel above spends ~30 % of its read
-// traffic on (2/BY of the rows here).  Loads are issued in the order they are consumed: this
-// plane's edge rows and x ends first, then plane z+2 and its tags, so the wait before the update
-// covers only loads from the previous iteration and the prefetch stays in flight across the
-// barrier.  No per-cell flag is read: every cell gets the bulk update of the phase with the most
+// rows, from global memory; the x neighbours at the ends of a lane's run come from the adjacent
+// lanes' registers (lane shuffles), from global memory only at the tile's two ends.  Only the
+// tile's edge rows and the z-chunk's two end planes are read from HBM a second time, which is what
+// the batched kernel above spends ~30 % of its read traffic on (2/BY of the rows here).  Loads are
+// issued in the order they are consumed: this plane's edge rows and tile-end x values first, then
+// plane z+2 and its tags, so the wait before the update covers only loads from the previous
+// iteration and the prefetch stays in flight across the barrier.  No per-cell flag is read: every cell gets the bulk update of the phase with the most
 // bulk cells (bulk_phase), and the cells whose flag is another value are listed and overwritten by
 // diffuse_general_kernel.  A cell moves only its value (and, mixed, its one-byte compensation
 // code): FP64 16 B, mixed 10 B per sub-step.  Same per-cell arithmetic and face order as
@@ -776,8 +776,12 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
       if (yp_g[r]) yp[r] = ldv(z, oyp[r]);
       else if (r + 1 < R) yp[r] = pc.c[r + 1 < R ? r + 1 : r];
       else yp[r] = sp[(tr + 1) * 64 + lane];
-      xm[r] = A(cin[plane * z + oxm[r]]);
-      xp[r] = A(cin[plane * z + oxp[r]]);
+      // x neighbours of the run's two ends: the adjacent lanes' registers, global memory only at
+      // the tile's ends and at the domain's last run (its +x neighbour wraps to x = 0)
+      xm[r] = __shfl_up(pc.c[r].v[V - 1], 1);
+      xp[r] = __shfl_down(pc.c[r].v[0], 1);
+      if (lane == 0) xm[r] = A(cin[plane * z + oxm[r]]);
+      if (lane == 63 || x0 + V >= nx) xp[r] = A(cin[plane * z + oxp[r]]);
     }
     // keep the prefetch below behind these loads: a wait for them then leaves it in flight
     __builtin_amdgcn_sched_barrier(0);
