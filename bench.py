@@ -72,14 +72,14 @@ def setup_fp64(m, p):
 # ---- byte models ----------------------------------------------------------------------------------
 def march_bytes_per_cell(mixed: bool) -> int:
     """Bytes the plane-marching kernel must move per cell and launch: the stored concentration read
-    once and written once (FP64 8 B each way; mixed FP32 value + one-byte compensation code)."""
-    return 5 + 5 if mixed else 8 + 8
+    once and written once (FP64 8 B each way; mixed FP32 value + 4-bit compensation code, 4.5 B)."""
+    return 9 if mixed else 8 + 8
 
 
-def listed_bytes_per_cell(mixed: bool, nphase: int) -> int:
+def listed_bytes_per_cell(mixed: bool, nphase: int) -> float:
     """Bytes of one listed cell's full-formula update with no reuse between cells: its own and its six
     neighbours' stored concentration and value_phase read, its concentration written."""
-    b_c, b_phi = (5, 4) if mixed else (8, 8)
+    b_c, b_phi = (4.5, 4) if mixed else (8, 8)
     return 7 * (b_c + nphase * b_phi) + b_c
 
 

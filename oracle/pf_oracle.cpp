@@ -93,10 +93,10 @@ template <> inline double as<double, Dual>(const Dual &x) { return x.v; }
 
 using std::sqrt;
 
-// Mixed form: the Kahan compensation is kept as an 8-bit signed fixed-point multiple of 2^-7 ulp
-// of the stored value (comp = q * 2^(e - 31), value = m * 2^e with 0.5 <= |m| < 1, q saturating at
-// [-128, 127]), rounded stochastically without bias: X = the compensation in units of 2^-24 of that
-// quantum (nearest integer, saturated to [-128, 127] quanta), U = the high 24 bits of a hash of the
+// Mixed form: the Kahan compensation is kept as a 4-bit signed fixed-point multiple of 2^-3 ulp of
+// the stored value (comp = q * 2^(e - 27), value = m * 2^e with 0.5 <= |m| < 1, q saturating at
+// [-8, 7]), rounded stochastically without bias: X = the compensation in units of 2^-24 of that
+// quantum (nearest integer, saturated to [-8, 7] quanta), U = the high 24 bits of a hash of the
 // cell index and the sub-step (key), q = floor((X + U) / 2^24) in integers.  This returns the
 // compensation that representation holds.
 inline uint32_t mix32(uint32_t x) {  // xorshift-multiply integer hash ("lowbias32" form)
@@ -107,14 +107,15 @@ inline uint32_t mix32(uint32_t x) {  // xorshift-multiply integer hash ("lowbias
   x ^= x >> 16;
   return x;
 }
+constexpr int kQuantumShift = 27;  // quantum 2^(e - 27)
 inline float quantize_comp(float comp, float value, uint32_t key) {
   int e = 0;
   (void)std::frexp(value, &e);
-  float s = std::ldexp(comp, 31 + 24 - e);
-  s = std::fmin(std::fmax(s, -2147483648.f), 2130706432.f);
+  float s = std::ldexp(comp, kQuantumShift + 24 - e);
+  s = std::fmin(std::fmax(s, -8.f * 16777216.f), 7.f * 16777216.f);
   const int64_t X = int64_t(std::nearbyint(s));
   const int64_t q = (X + int64_t(key >> 8)) >> 24;  // floor: arithmetic shift of a signed value
-  return std::ldexp(float(q), e - 31);
+  return std::ldexp(float(q), e - kQuantumShift);
 }
 // the mean of quantize_comp over all 2^24 thresholds (exact in int64)
 inline double quantize_comp_mean(float comp, float value) {
@@ -122,8 +123,8 @@ inline double quantize_comp_mean(float comp, float value) {
   int e = 0;
   (void)std::frexp(value, &e);
   for (uint32_t u = 0; u < (1u << 24); ++u)
-    sum += int64_t(std::ldexp(quantize_comp(comp, value, u << 8), 31 - e));
-  return std::ldexp(double(sum), e - 31 - 24);
+    sum += int64_t(std::ldexp(quantize_comp(comp, value, u << 8), kQuantumShift - e));
+  return std::ldexp(double(sum), e - kQuantumShift - 24);
 }
 template <class T> inline T quantize_comp(T comp, T, uint32_t) { return comp; }
 

@@ -732,11 +732,11 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         const unsigned *keep = vec && mby ? sd.count : nullptr;
         hipLaunchKernelGGL(adj_list_flag_kernel, dim3(gb), dim3(256), 0, st, sd.flag, N);
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
-                           ablock);
+                           ablock, ListOrder{}, static_cast<int *>(nullptr));
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl),
                            static_cast<int *>(nullptr));
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
-                           ablock, alist);
+                           ablock, alist, ListOrder{}, static_cast<int *>(nullptr));
       }
       const int mtx = (v.nx + 127) / 128, mty = mby ? (v.ny + mby - 1) / mby : 0;
       const long mnb = mby ? long(mtx) * mty * ((v.nz + mzc - 1) / mzc) : 0;

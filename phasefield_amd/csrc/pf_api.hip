@@ -37,7 +37,8 @@ struct pf_handle {
   pf_desc d{};
   long N = 0;
   int KI = 4, NF = 6, dims = 3;
-  size_t tsz = 8, csz = 8;  // bytes of one phi/value_phase element and one stored concentration
+  size_t tsz = 8;   // bytes of one phi/value_phase element
+  size_t cbuf = 0;  // bytes of one concentration buffer (FP64: 8 N; mixed: mixed_conc_bytes(N))
   hipStream_t stream = nullptr;
   uint16_t *id[2] = {nullptr, nullptr};
   void *phi[2] = {nullptr, nullptr};
@@ -76,10 +77,12 @@ struct pf_handle {
   int march_zc = 0;
   int *list_len = nullptr;   // [0]: cells the last sweep listed, [1]: cells the diffusion lists
   bool time_march = false;   // time the next plane-marching launch (timing kind 2)
-  int *glist = nullptr;      // indices of the listed cells, storage order
+  int *glist = nullptr;      // indices of the listed cells (sweep: row tiles, diffusion: storage)
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
   bool sweep_split = true;   // copy cells classified and written apart from the listed sweep
   uint8_t *sflag = nullptr;  // per cell: 1 = copied by the last sweep
+  int sweep_rows = -1;       // row tiles of the sweep's list (0: storage order, -1: by size)
+  int num_cu = 0;            // compute units of the handle's device
   bool sweep_hist = false;   // sflag describes the last sweep of the current buffers
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
   // The reference stores the partitioned concentrations (value_conc1) and reads them until they are
@@ -246,6 +249,16 @@ struct Timer {
   }
 };
 
+// Order of the sweep's list: row tiles of about 4096 cells per tile-plane (nx = 512: 8 rows), so
+// that the -z/+z neighbour rows of the cells a walk is on were touched a tile-plane earlier and are
+// still in L2; storage order for one plane (2D) or when PFAMD_SWEEP_ROWS=0.
+inline ListOrder sweep_list_order(const pf_handle *h) {
+  int rows = h->sweep_rows;
+  if (rows < 0) rows = std::max(1, 4096 / std::max(h->d.nx, 1));
+  if (h->d.nz == 1) rows = 0;
+  return ListOrder{h->d.nx, h->d.ny, h->d.nz, std::min(rows, h->d.ny)};
+}
+
 template <class F, int K, int P, int NF, bool NOF, bool ISO>
 hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
   using T = typename F::T;
@@ -262,14 +275,28 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     hipLaunchKernelGGL((sweep_classify_kernel<F, K, NF, NOF>), dim3(nb), dim3(256), 0, h->stream,
                        h->id[a], phi1, h->id[b], phi2, h->sflag, h->sweep_hist ? 1 : 0,
                        h->p.cutoff, h->d.nx, h->d.ny, h->d.nz, h->N);
+    const ListOrder order = sweep_list_order(h);
     hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
-                       static_cast<const unsigned *>(nullptr), h->N, h->gblock);
+                       static_cast<const unsigned *>(nullptr), h->N, h->gblock, order,
+                       static_cast<int *>(nullptr));
     hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock, int(nl),
                        h->list_len);
     hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
-                       static_cast<const unsigned *>(nullptr), h->N, h->gblock, h->glist);
-    // grid-stride over the list: enough workgroups to fill every CU at the LDS-limited occupancy
-    const unsigned ng = std::min<unsigned>(blocks_for(h->N, kSweepBS), 4096);
+                       static_cast<const unsigned *>(nullptr), h->N, h->gblock, h->glist, order,
+                       static_cast<int *>(nullptr));
+    // Every workgroup resident at once (the device's CUs times the kernel's occupancy), so that
+    // each XCD's workgroups walk their eighth of the list as one front of consecutive entries.
+    static int per_cu = 0;  // one value per instantiation
+    if (per_cu == 0) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+              &n, pf_sweep_kernel<F, K, P, NF, NOF, false, true>, kSweepBS, 0) != hipSuccess ||
+          n < 1)
+        n = 1;
+      per_cu = n;
+    }
+    const unsigned ng = unsigned(std::min<long>(blocks_for(h->N, kSweepBS),
+                                                long(std::max(h->num_cu, 1)) * per_cu));
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true>), dim3(ng), dim3(kSweepBS),
                        0, h->stream, h->id[a], phi1, h->id[b], phi2, static_cast<T *>(h->pfrac),
                        c0, cs, h->d.nx, h->d.ny, h->d.nz, h->N, h->glist, h->gblock + nl);
@@ -458,12 +485,17 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
         // bulk kernels leave the flag-0 cells
         const unsigned *keep = march ? h->bulk_count : nullptr;
         const unsigned nl = list_blocks(h->N);
+        // mixed: entries are byte pairs of cells (their codes share a byte); list_len[1] counts
+        // the listed cells either way
+        ListOrder order;
+        order.pairs = F::kMixed;
+        int *cells = F::kMixed ? h->list_len + 1 : nullptr;
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, keep,
-                           h->N, h->gblock);
+                           h->N, h->gblock, order, cells);
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock,
-                           int(nl), h->list_len + 1);
+                           int(nl), F::kMixed ? static_cast<int *>(nullptr) : h->list_len + 1);
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag,
-                           keep, h->N, h->gblock, h->glist);
+                           keep, h->N, h->gblock, h->glist, order, cells);
       }
       h->flag_valid = true;
       h->flag_nf = NF;
@@ -676,11 +708,16 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   auto h = std::make_unique<pf_handle>();
   h->d = d;
   h->N = long(d.nx) * d.ny * d.nz;
+  if (hipDeviceGetAttribute(&h->num_cu, hipDeviceAttributeMultiprocessorCount, d.device) !=
+          hipSuccess ||
+      h->num_cu < 1)
+    h->num_cu = 1;
   h->KI = d.depth == 4 ? 4 : 8;
   h->dims = dims;
   h->NF = 2 * dims;
   h->tsz = d.precision == PF_MIXED ? 4 : 8;
-  h->csz = d.precision == PF_MIXED ? 5 : 8;  // double, or float value + one-byte compensation code
+  // doubles, or float values followed by the 4-bit compensation codes
+  h->cbuf = d.precision == PF_MIXED ? size_t(mixed_conc_bytes(h->N)) : size_t(h->N) * 8;
   if (const char *e = std::getenv("PFAMD_DIFF_ZCHUNK")) h->diff_zchunk_override = std::atoi(e);
   if (const char *e = std::getenv("PFAMD_DIFF_KERNEL"))
     h->diff_variant = std::strcmp(e, "lds") == 0 ? 1 : 0;
@@ -690,6 +727,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
   if (const char *e = std::getenv("PFAMD_DIFF_VEC")) h->diff_vec = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_DIFF_MARCH")) h->diff_march = std::atoi(e) != 0;
   if (const char *e = std::getenv("PFAMD_SWEEP_SPLIT")) h->sweep_split = std::atoi(e) != 0;
+  if (const char *e = std::getenv("PFAMD_SWEEP_ROWS")) h->sweep_rows = std::atoi(e);
   pf_handle *hp = h.get();
   auto chk = [&](hipError_t e, const char *what) -> int {
     if (e == hipSuccess) return PF_OK;
@@ -705,7 +743,7 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     if ((rc = chk(hipMalloc(&h->id[b], N * h->KI * 2), "alloc ids")) != PF_OK) return rc;
     if ((rc = chk(hipMalloc(&h->phi[b], N * d.depth * h->tsz), "alloc phi")) != PF_OK) return rc;
     for (int s = 0; s < d.nsolute; ++s)
-      if ((rc = chk(hipMalloc(&h->conc[s][b], N * h->csz), "alloc conc")) != PF_OK) return rc;
+      if ((rc = chk(hipMalloc(&h->conc[s][b], h->cbuf), "alloc conc")) != PF_OK) return rc;
   }
   if ((rc = chk(hipMalloc(&h->pfrac, N * d.nphase * h->tsz), "alloc value_phase")) != PF_OK)
     return rc;
@@ -871,14 +909,14 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
     HIPCHK(h, hipMemcpyAsync(h->phi[0], v.data(), NK * 4, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->pfrac, ph.data(), ph.size() * 4, hipMemcpyHostToDevice, h->stream));
     for (int sol = 0; sol < h->d.nsolute; ++sol) {
-      // [value][compensation code] per buffer
-      std::vector<float> c(size_t(N) + (size_t(N) + 3) / 4);
-      CompCode *q = comp_codes(c.data(), N);
+      // [values][compensation codes] per buffer
+      std::vector<float> c(h->cbuf / 4, 0.f);
       for (long n = 0; n < N; ++n) {
         c[n] = float(s->conc[sol * N + n]);
-        q[n] = s->conc_comp ? comp_encode(float(s->conc_comp[sol * N + n]), c[n], 0u) : CompCode(0);
+        if (s->conc_comp)
+          comp_put_host(c.data(), N, n, comp_encode(float(s->conc_comp[sol * N + n]), c[n], 0u));
       }
-      HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), N * h->csz, hipMemcpyHostToDevice,
+      HIPCHK(h, hipMemcpyAsync(h->conc[sol][0], c.data(), h->cbuf, hipMemcpyHostToDevice,
                                h->stream));
       HIPCHK(h, hipStreamSynchronize(h->stream));
     }
@@ -939,13 +977,11 @@ int pf_download_state(pf_handle *h, pf_host_state *s) {
     for (long n = 0; n < N; ++n)
       for (int p = 0; p < P; ++p) phase[size_t(n) * 3 + p] = ph[size_t(n) * P + p];
     for (int sol = 0; sol < h->d.nsolute; ++sol) {
-      std::vector<float> c(size_t(N) + (size_t(N) + 3) / 4);
-      HIPCHK(h, hipMemcpy(c.data(), h->conc[sol][h->ccur[sol]], N * h->csz,
-                          hipMemcpyDeviceToHost));
-      const CompCode *q = comp_codes(c.data(), N);
+      std::vector<float> c(h->cbuf / 4);
+      HIPCHK(h, hipMemcpy(c.data(), h->conc[sol][h->ccur[sol]], h->cbuf, hipMemcpyDeviceToHost));
       for (long n = 0; n < N; ++n) {
         conc[sol * N + n] = c[n];
-        comp[sol * N + n] = comp_decode(q[n], c[n]);
+        comp[sol * N + n] = comp_decode(comp_load(c.data(), N, n), c[n]);
       }
     }
   } else {
@@ -1152,14 +1188,14 @@ int pf_list_lengths(pf_handle *h, int64_t *sweep_listed, int64_t *diffusion_list
 // restore into any handle maps codes back to the same ids), then the device buffers of the current
 // state in the device layout.
 namespace {
-constexpr char kCkptMagic[9] = "PFCKPT03";
+constexpr char kCkptMagic[9] = "PFCKPT04";
 constexpr size_t kCodeCap = 16384;  // serials 0..16383 of a 14-bit grain code
 constexpr size_t kCkptHeader = 64;
 constexpr size_t kCkptTables = 4 * sizeof(int32_t) + 3 * kCodeCap * sizeof(int32_t);
 size_t checkpoint_bytes(const pf_handle *h) {
   const size_t N = size_t(h->N);
   return kCkptHeader + kCkptTables + N * h->KI * 2 + N * h->d.depth * h->tsz +
-         N * h->d.nphase * h->tsz + N * h->csz * h->d.nsolute;
+         N * h->d.nphase * h->tsz + h->cbuf * h->d.nsolute;
 }
 }  // namespace
 
@@ -1203,8 +1239,8 @@ int pf_checkpoint_save(pf_handle *h, void *dst) {
   HIPCHK(h, hipMemcpy(p, h->pfrac, N * h->d.nphase * h->tsz, hipMemcpyDeviceToHost));
   p += N * h->d.nphase * h->tsz;
   for (int s = 0; s < h->d.nsolute; ++s) {
-    HIPCHK(h, hipMemcpy(p, h->conc[s][h->ccur[s]], N * h->csz, hipMemcpyDeviceToHost));
-    p += N * h->csz;
+    HIPCHK(h, hipMemcpy(p, h->conc[s][h->ccur[s]], h->cbuf, hipMemcpyDeviceToHost));
+    p += h->cbuf;
   }
   return PF_OK;
 }
@@ -1245,8 +1281,8 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
   HIPCHK(h, hipMemcpy(h->pfrac, p, N * h->d.nphase * h->tsz, hipMemcpyHostToDevice));
   p += N * h->d.nphase * h->tsz;
   for (int s = 0; s < h->d.nsolute; ++s) {
-    HIPCHK(h, hipMemcpy(h->conc[s][h->ccur[s]], p, N * h->csz, hipMemcpyHostToDevice));
-    p += N * h->csz;
+    HIPCHK(h, hipMemcpy(h->conc[s][h->ccur[s]], p, h->cbuf, hipMemcpyHostToDevice));
+    p += h->cbuf;
   }
   std::memcpy(h->csub, csub, sizeof(csub));
   for (int ph = 0; ph < 4; ++ph) h->code_to_id[ph] = tables[ph];

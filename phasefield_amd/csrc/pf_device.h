@@ -5,8 +5,9 @@
 //   slot phi    T      [N][K]    double (FP64 form) or float (mixed form), double-buffered
 //   phase frac  T      [N][P]    value_phase, P = 2 or 3 phases carried
 //   conc        C      [S][N]    per solute, double-buffered (Jacobi sub-steps):
-//                                FP64: double [N];  mixed: float value [N] followed by the one-byte
-//                                Kahan compensation code [N] in the same buffer (5 B per cell)
+//                                FP64: double [N];  mixed: float value [N] followed by the 4-bit
+//                                Kahan compensation codes, two cells per byte, in the same
+//                                buffer (4.5 B per cell)
 // A slot id is a 16-bit grain code: bits 15..14 = phase (1..3), bits 13..0 = grain serial;
 // 0 = empty slot.  The host maps the reference's int32 grain ids (100000+n, 200000+g) to codes
 // and back; the step only ever compares codes for equality, so results do not depend on it.
@@ -37,22 +38,24 @@ struct FormMixed {
 
 template <class C> __host__ __device__ inline C conc_hi(C c) { return c; }
 
-// Mixed-form Kahan compensation, stored in one signed byte per cell: comp = q * 2^(e - 31) where
-// c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. a multiple of 2^-7 ulp(c) (ulp(c) = 2^(e - 24)).
-// |comp| <= ulp/2 after a Kahan update, so |q| <= 64 with 2x headroom; q saturates at [-128, 127].
-// The effective concentration is c - comp.
+// Mixed-form Kahan compensation, stored in four bits per cell: comp = q * 2^(e - 27) where
+// c = m * 2^e (frexp, 0.5 <= |m| < 1), i.e. a multiple of 2^-3 ulp(c) (ulp(c) = 2^(e - 24)).
+// |comp| <= ulp/2 after a Kahan update, so |q| <= 4 with 2x headroom; q saturates at [-8, 7].
+// The effective concentration is c - comp.  The codes of a buffer of n cells follow its n float
+// values, two cells per byte: cell i in bits 4 * (i & 1) of byte i / 2.
 //
 // q is rounded stochastically and without bias: with X = comp in units of 2^-24 quanta (rounded to
-// the nearest integer, saturated to [-128, 127] quanta) and a threshold U = the high 24 bits of a
+// the nearest integer, saturated to [-8, 7] quanta) and a threshold U = the high 24 bits of a
 // per-cell hash of the sub-step, q = floor((X + U) / 2^24), computed in integers.  Over uniformly
 // distributed U the mean of q is X / 2^24 exactly.  (Thresholds with fewer fraction bits than the
-// compensation round down more often than up: with 16-bit thresholds the stored compensation was
-// low by up to 2^-16 quanta in every cell with a nonzero increment, a systematic mass gain that
-// grew as the diffusion front reached more cells.)  Rounding to nearest would drop every increment
-// below half a quantum, and those losses do not cancel between neighbours of different exponents.
-// A compensation already on the grid is kept by every threshold; threshold 0 rounds down.
-using CompCode = int8_t;
-constexpr int kCompShift = 31;
+// compensation round down more often than up: 16-bit thresholds leave the stored compensation low
+// by up to 2^-16 quanta in every cell with a nonzero increment, a systematic mass gain that grows
+// as the diffusion front reaches more cells.)  Rounding to nearest would drop every increment below
+// half a quantum, and those losses do not cancel between neighbours of different exponents.  A
+// compensation already on the grid is kept by every threshold; threshold 0 rounds down.
+using CompCode = int8_t;  // q, sign-extended from its four stored bits
+constexpr int kCompShift = 27;
+constexpr int kCompMin = -8, kCompMax = 7;
 __host__ __device__ inline int comp_exp(float c) {
   int e = 0;
   (void)frexpf(c, &e);
@@ -60,6 +63,10 @@ __host__ __device__ inline int comp_exp(float c) {
 }
 __host__ __device__ inline float comp_decode(CompCode code, float c) {
   return ldexpf(float(code), comp_exp(c) - kCompShift);
+}
+// q of four stored bits (the low nibble of `bits`)
+__host__ __device__ inline CompCode comp_unpack(uint32_t bits) {
+  return CompCode(int32_t(bits << 28) >> 28);
 }
 // 32-bit integer mixer (xorshift-multiply rounds of the "lowbias32" form)
 __host__ __device__ inline uint32_t mix32(uint32_t x) {
@@ -85,24 +92,38 @@ __host__ __device__ inline uint32_t comp_hash(uint32_t i, uint32_t key) {
 // compensation already on the grid, as in uploads)
 __host__ __device__ inline CompCode comp_encode(float comp, float c, uint32_t h) {
   float s = ldexpf(comp, kCompShift + 24 - comp_exp(c));
-  s = fminf(fmaxf(s, -2147483648.f), 2130706432.f);  // [-128, 127] quanta in units of 2^-24
+  s = fminf(fmaxf(s, float(kCompMin) * 16777216.f), float(kCompMax) * 16777216.f);
   const int X = int(rintf(s));
-  return CompCode((X + int(h >> 8)) >> 24);  // floor((X + U) / 2^24), never beyond [-128, 127]
+  return CompCode((X + int(h >> 8)) >> 24);  // floor((X + U) / 2^24), never beyond [-8, 7]
+}
+// bytes of a mixed concentration buffer of n cells: n floats, then the codes, padded to 16 B
+__host__ __device__ inline long mixed_conc_bytes(long n) {
+  return 4 * n + ((n + 1) / 2 + 15) / 16 * 16;
+}
+// the code bytes of a mixed concentration buffer of n cells
+__host__ __device__ inline const uint8_t *comp_bytes(const float *buf, long n) {
+  return reinterpret_cast<const uint8_t *>(buf + n);
+}
+__host__ __device__ inline uint8_t *comp_bytes(float *buf, long n) {
+  return reinterpret_cast<uint8_t *>(buf + n);
+}
+// q of cell i
+__host__ __device__ inline CompCode comp_load(const float *buf, long n, long i) {
+  return comp_unpack(uint32_t(comp_bytes(buf, n)[i >> 1]) >> (4 * int(i & 1)));
+}
+// host side: store q of cell i (no other writer of its byte at the same time)
+inline void comp_put_host(float *buf, long n, long i, CompCode q) {
+  uint8_t &b = comp_bytes(buf, n)[i >> 1];
+  const int sh = 4 * int(i & 1);
+  b = uint8_t((b & ~(15u << sh)) | ((uint32_t(q) & 15u) << sh));
 }
 // effective stored concentration of cell i of a buffer of n cells (mixed: value - compensation)
 template <class F> __device__ inline double conc_effective(const typename F::C *buf, long i, long n);
 template <> __device__ inline double conc_effective<FormF64>(const double *buf, long i, long) {
   return buf[i];
 }
-// compensation codes of a mixed concentration buffer of n cells
-__host__ __device__ inline const CompCode *comp_codes(const float *buf, long n) {
-  return reinterpret_cast<const CompCode *>(buf + n);
-}
-__host__ __device__ inline CompCode *comp_codes(float *buf, long n) {
-  return reinterpret_cast<CompCode *>(buf + n);
-}
 template <> __device__ inline double conc_effective<FormMixed>(const float *buf, long i, long n) {
-  return double(buf[i]) - double(comp_decode(comp_codes(buf, n)[i], buf[i]));
+  return double(buf[i]) - double(comp_decode(comp_load(buf, n, i), buf[i]));
 }
 
 // (x, y, z) of a storage-order cell index.  Cell indices stay below 2^31 (pf_create rejects larger

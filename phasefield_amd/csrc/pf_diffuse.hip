@@ -505,10 +505,20 @@ template <class T, int V> __device__ inline void store_nt(VecA<T, V> *p, const V
 #pragma unroll
   for (int k = 0; k < V; ++k) __builtin_nontemporal_store(v.v[k], &p->v[k]);
 }
-template <int V> __device__ inline void store_nt(VecCode<V> *p, const VecCode<V> &q) {
-  static_assert(V == 4, "four one-byte codes per word");
-  __builtin_nontemporal_store(*reinterpret_cast<const uint32_t *>(&q),
-                              reinterpret_cast<uint32_t *>(p));
+// codes of the four cells i..i+3 (i a multiple of 4): one 16-bit load, four bits per cell
+__device__ inline VecCode<4> load_codes4(const float *__restrict__ buf, long n, long i) {
+  const uint32_t w = *reinterpret_cast<const uint16_t *>(comp_bytes(buf, n) + (i >> 1));
+  VecCode<4> q;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) q.q[v] = comp_unpack(w >> (4 * v));
+  return q;
+}
+__device__ inline void store_codes4_nt(float *__restrict__ buf, long n, long i,
+                                       const VecCode<4> &q) {
+  uint32_t w = 0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) w |= (uint32_t(q.q[v]) & 15u) << (4 * v);
+  __builtin_nontemporal_store(uint16_t(w), reinterpret_cast<uint16_t *>(comp_bytes(buf, n) + (i >> 1)));
 }
 
 template <class F, int V>
@@ -546,7 +556,7 @@ __device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, fl
     qo.q[v] = comp_encode((t - c.v[v]) - y, t, h[v]);
   }
   store_nt(reinterpret_cast<VecA<float, 4> *>(out + i), o);
-  store_nt(reinterpret_cast<VecCode<4> *>(comp_codes(out, n) + i), qo);
+  store_codes4_nt(out, n, i, qo);
 }
 
 template <class F, int P, int NF, bool NOFLUX, int BY, int V>
@@ -590,8 +600,7 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
     for (int u = 0; u < U; ++u) {
       const long base = plane * min(z + u, z1 - 1);
       const typename F::C *__restrict__ pl = cin + base;
-      if constexpr (F::kMixed)
-        qc[u] = *reinterpret_cast<const VecCode<V> *>(comp_codes(cin, ncell) + base + oc);
+      if constexpr (F::kMixed) qc[u] = load_codes4(cin, ncell, base + oc);
       fl[u] = *reinterpret_cast<const VecFlag<V> *>(flag + base + oc);
       cc[u] = ldv(pl, oc);
       xm[u] = A(pl[oxm]);
@@ -660,12 +669,12 @@ __device__ inline void lds_barrier() {
 // the batched kernel above spends ~30 % of its read traffic on (2/BY of the rows here).  Loads are
 // issued in the order they are consumed: this plane's edge rows and tile-end x values first, then
 // plane z+2 and its tags, so the wait before the update covers only loads from the previous
-// iteration and the prefetch stays in flight across the barrier.  No per-cell flag is read: every cell gets the bulk update of the phase with the most
-// bulk cells (bulk_phase), and the cells whose flag is another value are listed and overwritten by
-// diffuse_general_kernel.  A cell moves only its value (and, mixed, its one-byte compensation
-// code): FP64 16 B, mixed 10 B per sub-step.  Same per-cell arithmetic and face order as
-// diffuse_bulk_vec_kernel (-x, +x, -y, +y, -z, +z), so the two are bitwise interchangeable on the
-// cells both keep.  Needs nx % V == 0; partial tiles (x or y) are masked but keep every barrier.
+// iteration and the prefetch stays in flight across the barrier.  No per-cell flag is read: every
+// cell gets the bulk update of the phase with the most bulk cells (bulk_phase), and the cells whose
+// flag is another value are listed and overwritten by diffuse_general_kernel.  A cell moves only
+// its value (and, mixed, its 4-bit compensation code): FP64 16 B, mixed 9 B per sub-step.  Same
+// per-cell arithmetic and face order as diffuse_bulk_vec_kernel (-x, +x, -y, +y, -z, +z), so the
+// two are bitwise interchangeable on the cells both keep.  Needs nx % V == 0; partial tiles (x or y) are masked but keep every barrier.
 template <class F, int P, int NF, bool NOFLUX, int BY, int V, int R>
 __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
@@ -675,7 +684,7 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
   using A = typename F::A;
   using VA = VecA<A, V>;
   static_assert(BY % R == 0, "rows per wavefront must divide the tile height");
-  // per-cell tags of a plane: the mixed form's V one-byte compensation codes as one raw word, kept
+  // per-cell tags of a plane: the mixed form's V 4-bit compensation codes as one raw word, kept
   // packed until the plane is updated (see the note in the plane step); none for FP64
   struct Tag {
     uint32_t w;
@@ -743,7 +752,8 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
     if constexpr (F::kMixed) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        d.t[r].w = *reinterpret_cast<const uint32_t *>(comp_codes(cin, ncell) + plane * z + oc[r]);
+        d.t[r].w = *reinterpret_cast<const uint16_t *>(comp_bytes(cin, ncell) +
+                                                       ((plane * z + oc[r]) >> 1));
     }
   };
   // Register rings, indexed by the plane's position in the chunk modulo 4: planes z-1, z, z+1 are
@@ -802,7 +812,7 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
         Tag tg = tc.t[r];
         asm volatile("" : "+v"(tg.w));
 #pragma unroll
-        for (int v = 0; v < V; ++v) qc.q[v] = CompCode(tg.w >> (8 * v));
+        for (int v = 0; v < V; ++v) qc.q[v] = comp_unpack(tg.w >> (4 * v));
       }
       if (act[r]) {
         A del[V];
@@ -890,7 +900,7 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
 
 // The listed cells (next to a phase boundary, and with the plane-marching kernel also the bulk
 // cells of the other phases), from the list built once per sweep: the full partition + weighted
-// flux, grid-stride over the device-side count.
+// flux, grid-stride over the device-side count of list entries.
 template <class F, int P, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void diffuse_general_kernel(
     const typename F::C *__restrict__ cin, typename F::C *__restrict__ cout,
@@ -911,9 +921,42 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
     nbi[4] = i + plane * (zm - z);
     nbi[5] = i + plane * (zp - z);
   };
-  // two list entries per iteration, the second's loads issued before the first's arithmetic (the
-  // pass is bound by the latency of its gathers, not by bandwidth)
   const int stride = gridDim.x * 256;
+  if constexpr (F::kMixed) {
+    // Mixed: the list's entries are byte pairs (ListOrder::pairs), j | mask << 30, and a thread
+    // owns the code byte of cells 2j and 2j + 1: their values and the byte are written with plain
+    // stores (a cell of the pair that is not listed keeps the code the bulk kernel wrote).  The
+    // loads of both cells are issued before the arithmetic.
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += stride) {
+      const uint32_t e = uint32_t(list[k]);
+      const long j = long(e & 0x3fffffffu);
+      const uint32_t m = e >> 30;
+      const long i0 = 2 * j + ((m & 1u) ? 0 : 1);  // the first listed cell of the pair
+      const long i1 = m == 3u ? i0 + 1 : i0;
+      long nb0[6], nb1[6];
+      stencil(i0, nb0);
+      stencil(i1, nb1);
+      StencilLoad<F, P, NF> s0, s1;
+      stencil_load<F, P, NF>(cin, pfrac, i0, ncell, nb0, s0);
+      stencil_load<F, P, NF>(cin, pfrac, i1, ncell, nb1, s1);
+      uint8_t *cb = comp_bytes(cout, ncell) + j;
+      uint32_t byte = m == 3u ? 0u : uint32_t(*cb);
+      CompCode q;
+      cout[i0] = mixed_step(s0.me.c, stencil_del<F, P, NF>(s0, dc), s0.q,
+                            comp_hash(uint32_t(i0), dc.key), q);
+      const int sh0 = 4 * int(i0 & 1);
+      byte = (byte & ~(15u << sh0)) | ((uint32_t(q) & 15u) << sh0);
+      if (m == 3u) {
+        cout[i1] = mixed_step(s1.me.c, stencil_del<F, P, NF>(s1, dc), s1.q,
+                              comp_hash(uint32_t(i1), dc.key), q);
+        byte = (byte & 15u) | ((uint32_t(q) & 15u) << 4);
+      }
+      *cb = uint8_t(byte);
+    }
+    return;
+  }
+  // FP64: one cell per entry; two entries per iteration, the second's loads issued before the
+  // first's arithmetic (the pass is bound by the latency of its gathers, not by bandwidth)
   for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += 2 * stride) {
     const bool two = k + stride < n;
     const long i0 = list[k], i1 = two ? long(list[k + stride]) : i0;
@@ -973,40 +1016,53 @@ __global__ __launch_bounds__(256) void diffuse_retable_kernel(
   conc_store<F>(cout, i, ncell, s.me.c, del, s.q, dc.key);
 }
 
-// ---- list of the cells that need the full formula, in storage order -----------------------------
+// ---- list of the cells that need the full formula ---------------------------------------------
 __device__ inline bool listed_cell(int f, const unsigned *count, int keep) {
   return count ? f != keep : f == 0;
 }
-// Blocks of 256 threads x kListChunk consecutive cells (one 16-byte load of flags per thread).
-// count: listed cells per block; scan: exclusive offsets (one workgroup); scatter: each thread
-// writes its listed cells in order at the block's offset plus the exclusive sum of the counts of
-// the threads before it, so the list is in storage order.
+// Blocks of 256 threads x kListChunk consecutive list-order positions (one 16-byte load of flags
+// per thread when the chunk's cells are consecutive in storage: always in storage order, and in
+// row-tile order when nx % kListChunk == 0, since every row then starts at a multiple of
+// kListChunk).  count: listed cells per block; scan: exclusive offsets (one workgroup); scatter:
+// each thread writes its listed cells in order at the block's offset plus the exclusive sum of the
+// counts of the threads before it, so the list follows the list order.
 static_assert(kListChunk == 16, "one uint4 of flags per thread");
-// listed bits of the thread's kListChunk cells starting at `first` (cells at or past ncell: none)
+__device__ inline bool list_chunk_contiguous(const ListOrder &o) {
+  return o.rows == 0 || o.nx % kListChunk == 0;
+}
+// listed bits of the thread's kListChunk positions starting at `first` (positions at or past
+// ncell: none)
 __device__ inline unsigned listed_bits(const uint8_t *__restrict__ flag, long first, long ncell,
-                                       const unsigned *count, int keep) {
+                                       const unsigned *count, int keep, const ListOrder &o) {
   unsigned bits = 0;
-  if (first + kListChunk <= ncell) {
-    const uint4 w = *reinterpret_cast<const uint4 *>(flag + first);
+  if (first + kListChunk <= ncell && list_chunk_contiguous(o)) {
+    const uint4 w = *reinterpret_cast<const uint4 *>(flag + list_order_cell(o, first));
     const uint32_t word[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int j = 0; j < kListChunk; ++j)
       if (listed_cell(int((word[j >> 2] >> (8 * (j & 3))) & 0xffu), count, keep)) bits |= 1u << j;
   } else {
     for (int j = 0; j < kListChunk && first + j < ncell; ++j)
-      if (listed_cell(flag[first + j], count, keep)) bits |= 1u << j;
+      if (listed_cell(flag[list_order_cell(o, first + j)], count, keep)) bits |= 1u << j;
   }
   return bits;
 }
 
+// the pairs (2j, 2j + 1) of a chunk's listed bits with a listed cell: bit 2j' set for pair j'
+__device__ inline unsigned pair_bits(unsigned bits) { return (bits | (bits >> 1)) & 0x5555u; }
+
 __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__restrict__ flag,
                                                             const unsigned *__restrict__ count,
-                                                            long ncell, int *__restrict__ bcount) {
+                                                            long ncell, int *__restrict__ bcount,
+                                                            const ListOrder order,
+                                                            int *__restrict__ cells) {
   __shared__ int wsum[4];
   const long first = long(blockIdx.x) * (256L * kListChunk) + long(threadIdx.x) * kListChunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int keep = count ? bulk_phase(count) : 0;
-  int n = __popc(listed_bits(flag, first, ncell, count, keep));
+  if (cells && blockIdx.x == 0 && threadIdx.x == 0) *cells = 0;  // the scatter kernel adds to it
+  const unsigned bits = listed_bits(flag, first, ncell, count, keep, order);
+  int n = __popc(order.pairs ? pair_bits(bits) : bits);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) n += __shfl_xor(n, d);
   if (lane == 0) wsum[wave] = n;
@@ -1047,12 +1103,40 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
                                                               const unsigned *__restrict__ count,
                                                               long ncell,
                                                               const int *__restrict__ boff,
-                                                              int *__restrict__ list) {
+                                                              int *__restrict__ list,
+                                                              const ListOrder order,
+                                                              int *__restrict__ cells) {
   __shared__ int wsum[4];
   const long first = long(blockIdx.x) * (256L * kListChunk) + long(threadIdx.x) * kListChunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int keep = count ? bulk_phase(count) : 0;
-  unsigned bits = listed_bits(flag, first, ncell, count, keep);
+  unsigned bits = listed_bits(flag, first, ncell, count, keep, order);
+  if (cells) {  // listed cells of the wave, one atomic per wave
+    int nc = __popc(bits);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) nc += __shfl_xor(nc, d);
+    if (lane == 0 && nc) atomicAdd(cells, nc);
+  }
+  if (order.pairs) {
+    unsigned pb = pair_bits(bits);
+    const int c = __popc(pb);
+    int v = c;  // inclusive scan over the wave's lanes
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(v, d);
+      if (lane >= d) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    int pos = boff[blockIdx.x] + v - c;
+    for (int w = 0; w < wave; ++w) pos += wsum[w];
+    while (pb) {
+      const int j = __builtin_ctz(pb);  // even: the pair's first cell
+      pb &= pb - 1;
+      list[pos++] = int(uint32_t((first + j) >> 1) | (((bits >> j) & 3u) << 30));
+    }
+    return;
+  }
   const int c = __popc(bits);
   int v = c;  // inclusive scan over the wave's lanes
 #pragma unroll
@@ -1064,10 +1148,19 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
   __syncthreads();
   int pos = boff[blockIdx.x] + v - c;
   for (int w = 0; w < wave; ++w) pos += wsum[w];
-  while (bits) {
-    const int j = __builtin_ctz(bits);
-    bits &= bits - 1;
-    list[pos++] = int(first + j);
+  if (bits && list_chunk_contiguous(order)) {
+    const long base = list_order_cell(order, first);
+    while (bits) {
+      const int j = __builtin_ctz(bits);
+      bits &= bits - 1;
+      list[pos++] = int(base + j);
+    }
+  } else {
+    while (bits) {
+      const int j = __builtin_ctz(bits);
+      bits &= bits - 1;
+      list[pos++] = int(list_order_cell(order, first + j));
+    }
   }
 }
 

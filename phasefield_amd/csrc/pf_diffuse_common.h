@@ -46,6 +46,16 @@ template <class A> __device__ inline A face_weight(A prod) {
   return w;
 }
 
+// Store q of cell i while other threads may store the other cells of its 32-bit word: each thread
+// changes only its own four bits, by an atomic XOR with the difference.  Its own bits were last
+// written by an earlier kernel (or are this thread's alone), so a plain load reads them.
+__device__ inline void comp_put_atomic(float *buf, long n, long i, CompCode q) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(comp_bytes(buf, n)) + (i >> 3);
+  const int sh = 4 * int(i & 7);
+  const uint32_t d = ((*w >> sh) ^ uint32_t(q)) & 15u;
+  if (d) atomicXor(w, d << sh);
+}
+
 // total_conc += del for cell i (solver.cpp:366) of stored value c, written to `out` (n cells per
 // buffer).  Mixed form: Kahan-compensated with the cell's compensation code q, y = del - comp;
 // t = c + y; comp = (t - c) - y; c = t, the new code stored after the n values of the buffer and
@@ -58,14 +68,20 @@ __device__ inline void conc_store<FormF64>(double *__restrict__ out, long i, lon
                                            double del, CompCode, uint32_t) {
   out[i] = c + del;
 }
-template <>
-__device__ inline void conc_store<FormMixed>(float *__restrict__ out, long i, long n, float c,
-                                             float del, CompCode q, uint32_t key) {
+// the mixed form's Kahan step: new value t and its code from value c, code q, increment del
+__device__ inline float mixed_step(float c, float del, CompCode q, uint32_t h, CompCode &qo) {
   const float comp = comp_decode(q, c);
   const float y = del - comp;
   const float t = c + y;
-  out[i] = t;
-  comp_codes(out, n)[i] = comp_encode((t - c) - y, t, comp_hash(uint32_t(i), key));
+  qo = comp_encode((t - c) - y, t, h);
+  return t;
+}
+template <>
+__device__ inline void conc_store<FormMixed>(float *__restrict__ out, long i, long n, float c,
+                                             float del, CompCode q, uint32_t key) {
+  CompCode qo;
+  out[i] = mixed_step(c, del, q, comp_hash(uint32_t(i), key), qo);
+  comp_put_atomic(out, n, i, qo);
 }
 // the same with the code read from the input buffer `in`
 template <class F>
@@ -73,7 +89,7 @@ __device__ inline void conc_update(const typename F::C *__restrict__ in,
                                    typename F::C *__restrict__ out, long i, long n,
                                    typename F::C c, typename F::A del, uint32_t key) {
   CompCode q = 0;
-  if constexpr (F::kMixed) q = comp_codes(in, n)[i];
+  if constexpr (F::kMixed) q = comp_load(in, n, i);
   conc_store<F>(out, i, n, c, del, q, key);
 }
 
@@ -98,13 +114,12 @@ __device__ inline void stencil_load(const typename F::C *__restrict__ cin,
     for (int p = 0; p < P; ++p) s.nb[f].ph[p] = pfrac[nbi[f] * P + p];
   }
   s.q = 0;
-  if constexpr (F::kMixed) s.q = comp_codes(cin, n)[i];
+  if constexpr (F::kMixed) s.q = comp_load(cin, n, i);
 }
-// partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365), update
+// partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365): the cell's increment
 template <class F, int P, int NF>
-__device__ inline void stencil_update(typename F::C *__restrict__ cout, long i, long n,
-                                      const StencilLoad<F, P, NF> &s,
-                                      const DiffConsts<typename F::A> &dc) {
+__device__ inline typename F::A stencil_del(const StencilLoad<F, P, NF> &s,
+                                            const DiffConsts<typename F::A> &dc) {
   using A = typename F::A;
   CellPF<A> me;
   partition_cell<F, P>(s.me, dc, me);
@@ -117,8 +132,14 @@ __device__ inline void stencil_update(typename F::C *__restrict__ cout, long i, 
     for (int p = 0; p < P; ++p)
       del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
   }
-  del *= dc.r;
-  conc_store<F>(cout, i, n, s.me.c, del, s.q, dc.key);
+  return del * dc.r;
+}
+// ... and the update
+template <class F, int P, int NF>
+__device__ inline void stencil_update(typename F::C *__restrict__ cout, long i, long n,
+                                      const StencilLoad<F, P, NF> &s,
+                                      const DiffConsts<typename F::A> &dc) {
+  conc_store<F>(cout, i, n, s.me.c, stencil_del<F, P, NF>(s, dc), s.q, dc.key);
 }
 
 // The reference's sub-step for one cell from its own and its stencil's stored concentration and
@@ -139,7 +160,7 @@ template <class A, int V> struct alignas(sizeof(A) * V) VecA {
 template <int V> struct alignas(V) VecFlag {
   uint8_t f[V];
 };
-template <int V> struct alignas(V) VecCode {
+template <int V> struct VecCode {  // decoded compensation codes of a run
   CompCode q[V];
 };
 

@@ -103,17 +103,48 @@ __global__ void diffuse_retable_kernel(const typename F::C *__restrict__ cin,
                                        const DiffConsts<typename F::A> dc_prev, const int nx,
                                        const int ny, const int nz);
 
-// List of the cells a full-formula pass must cover, in storage order: flag == 0, or, with a
-// non-null `count`, flag != bulk_phase(count) (the cells the plane-marching kernel gets wrong).
+// List of the cells a full-formula pass must cover: flag == 0, or, with a non-null `count`,
+// flag != bulk_phase(count) (the cells the plane-marching kernel gets wrong).
 constexpr int kListChunk = 16;  // 256-cell chunks per block of the full-formula list build
+// Order of the list.  rows == 0: storage order.  rows > 0: row tiles — `rows` full-width rows
+// (the last tile may hold fewer), a tile's planes one after another, the tiles in y order — so a
+// walk over the list meets a cell's -z and +z neighbour rows one tile-plane (rows * nx cells)
+// before and after the cell instead of one whole plane.  Each listed cell's update reads only the
+// source buffers, so the order changes which rows share the cache, never a result.
+// pairs (storage order only): one entry per byte pair of cells (2j, 2j + 1) holding a listed
+// cell, j | mask << 30 with mask bit b set when cell 2j + b is listed (cell indices stay below
+// 2^31).  The mixed form's 4-bit compensation codes of the two cells share a byte, so a pass that
+// gives a pair to one thread writes that byte with a plain store.
+struct ListOrder {
+  int nx = 0, ny = 0, nz = 0, rows = 0;
+  bool pairs = false;
+};
+// storage index of list-order position p (0 <= p < nx * ny * nz)
+__host__ __device__ inline long list_order_cell(const ListOrder &o, long p) {
+  if (o.rows == 0) return p;
+  const long column = long(o.rows) * o.nx * o.nz;  // cells of one full tile
+  const int ty = int(p / column);
+  const int h = o.ny - ty * o.rows < o.rows ? o.ny - ty * o.rows : o.rows;
+  const long q = p - ty * column;
+  const long plane = long(h) * o.nx;
+  const int z = int(q / plane);
+  const long r = q - z * plane;
+  const int yy = int(r / o.nx);
+  const int x = int(r - long(yy) * o.nx);
+  return (long(z) * o.ny + long(ty) * o.rows + yy) * o.nx + x;
+}
+// `cells` (optional): the count kernel sets it to 0 and the scatter kernel adds the listed cells to
+// it, for lists whose entries are not cells (pairs)
 __global__ void general_count_kernel(const uint8_t *__restrict__ flag,
                                      const unsigned *__restrict__ count, long ncell,
-                                     int *__restrict__ bcount);
+                                     int *__restrict__ bcount, const ListOrder order,
+                                     int *__restrict__ cells);
 __global__ void general_scan_kernel(int *__restrict__ bcount, int nb,
                                     int *__restrict__ total_out);
 __global__ void general_scatter_kernel(const uint8_t *__restrict__ flag,
                                        const unsigned *__restrict__ count, long ncell,
-                                       const int *__restrict__ boff, int *__restrict__ list);
+                                       const int *__restrict__ boff, int *__restrict__ list,
+                                       const ListOrder order, int *__restrict__ cells);
 
 // Bulk flags of a value_phase field into `flag` and the bulk cells per phase into count[1..3]
 // (two kernels; `unit` is an N-byte scratch array)

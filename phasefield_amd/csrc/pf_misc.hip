@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void convert_conc_f64_to_mixed(const double *_
   const long i = long(blockIdx.x) * 256 + threadIdx.x;
   if (i < n) {
     dst[i] = float(src[i]);
-    comp_codes(dst, n)[i] = 0;
+    if (i < (n + 1) / 2) comp_bytes(dst, n)[i] = 0;  // the code bytes of cells 2i, 2i + 1
   }
 }
 

@@ -261,9 +261,10 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
   };
   if (LIST) {
-    // Workgroup b runs on XCD b % 8: each XCD walks a contiguous eighth of the (storage-order)
-    // list, so the neighbour rows of a workgroup's cells are more often in its own XCD's L2.
-    // (Fewer than 8 workgroups: one plain grid-stride walk, so that every range has a walker.)
+    // Workgroup b runs on XCD b % 8: each XCD walks a contiguous eighth of the list (row-tile
+    // order, all its workgroups resident, one front of consecutive entries), so the neighbour
+    // rows of a workgroup's cells are in its own XCD's L2.  (Fewer than 8 workgroups: one plain
+    // grid-stride walk, so that every range has a walker.)
     const int n = *count;
     if (gridDim.x < 8) {
       for (int k = blockIdx.x * kSweepBS + t; k < n; k += gridDim.x * kSweepBS) body(list[k]);

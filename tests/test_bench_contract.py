@@ -22,10 +22,10 @@ def test_more_than_one_gpu_is_refused():
 def test_byte_models():
     # plane-marching kernel: the stored concentration read and written once per cell
     assert bench.march_bytes_per_cell(False) == 8 + 8
-    assert bench.march_bytes_per_cell(True) == 5 + 5  # FP32 value + one-byte code each way
+    assert bench.march_bytes_per_cell(True) == 4.5 + 4.5  # FP32 value + 4-bit code each way
     # a listed cell: its own and six neighbours' concentration + value_phase, one write
     assert bench.listed_bytes_per_cell(False, 2) == 7 * (8 + 16) + 8
-    assert bench.listed_bytes_per_cell(True, 2) == 7 * (5 + 8) + 5
+    assert bench.listed_bytes_per_cell(True, 2) == 7 * (4.5 + 8) + 4.5
     # the scope document's model per step at 200 sub-steps, K = 4, P = 2, Mn once per step
     assert bench.model_8d_bytes_per_cell_step(False, 2, 2, 200) == 6560
     assert bench.model_8d_bytes_per_cell_step(True, 2, 2, 200) == 4904
@@ -37,8 +37,8 @@ def test_measured_traffic_names_its_sources():
     assert t["stale"] == (t.get("sources") != bench.kernel_source_hash())
     if not t["stale"]:
         m = t["march"]
-        # at least the compulsory bytes the marching kernel moves per cell (16 / 10 B at 512^3)
-        assert m["FormF64"] >= 16 * 512 ** 3 and m["FormMixed"] >= 10 * 512 ** 3
+        # at least the compulsory bytes the marching kernel moves per cell (16 / 9 B at 512^3)
+        assert m["FormF64"] >= 16 * 512 ** 3 and m["FormMixed"] >= 9 * 512 ** 3
 
 
 def test_pmc_summary_reads_counter_files(tmp_path):

@@ -121,6 +121,23 @@ def test_full_steps_bitwise(tmp_path, monkeypatch, shape, bc, K, sweep_split):
 
 
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("rows", ["0", "1", "5", "-1"])
+@pytest.mark.parametrize("shape", [(70, 20, 18), (64, 23, 18)])
+def test_sweep_list_orders_bitwise(tmp_path, monkeypatch, shape, rows, form):
+    """The sweep's list in storage order (rows 0) and in row tiles of 1, 5 and the size-chosen
+    number of rows: 23 rows leave a 3-row last tile, nx = 70 takes the per-position list build
+    (rows not starting at multiples of 16 positions) and nx = 64 the 16-flag loads.  The cells'
+    updates are independent, so every order gives the oracle's result bitwise."""
+    banner(f"sweep list order rows={rows} shape={shape} form={form}")
+    monkeypatch.setenv("PFAMD_SWEEP_ROWS", rows)
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=6, nuclei=4)
+    gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p,
+                       lambda g, q: g.step(3), lambda o, q: o.step(3))
+    assert_same(gs, os_, 4, f"sweep list rows={rows}")
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
 @pytest.mark.parametrize("kernel,rows,split,vec,march,zchunk", [
     ("bulk", "-1", "1", "1", "1", "0"),
     ("bulk", "8", "1", "1", "1", "5"), ("bulk", "16", "1", "1", "1", "3"),

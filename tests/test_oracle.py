@@ -337,8 +337,8 @@ def test_noflux_equals_mirrored_periodic(params3d, rng):
 
 
 def test_compensation_rounding_is_unbiased_and_exact():
-    """The mixed form's one-byte Kahan compensation (multiples of 2^-7 ulp of the value, saturating
-    at -128..127) is rounded stochastically with thresholds from hashed keys.
+    """The mixed form's 4-bit Kahan compensation (multiples of 2^-3 ulp of the value, saturating
+    at -8..7) is rounded stochastically with thresholds from hashed keys.
     - Over all 2^24 thresholds the mean of the stored compensation equals the compensation exactly
       (exhaustive, in integers), also for compensations with bits far below a quantum; thresholds
       with 16 fraction bits would give floor(f 2^16)/2^16 of a quantum for the fraction f instead.
@@ -350,26 +350,27 @@ def test_compensation_rounding_is_unbiased_and_exact():
     q = lambda comp, value, key: L.orc_quantize_comp(comp, value, key)
     keys = [L.orc_mix32(i) for i in range(1 << 15)]
     for value, ms in ((0.02, (0x4CCCCD + 1, -0x2AAAAB, 3)),
-                      (1.7, (0x4CCCCD, 5 * (1 << 24) + 0x34568, -(41 << 24) - (7 << 6)))):
+                      (1.7, (0x4CCCCD, 5 * (1 << 24) + 0x34568, -(7 << 24) - (7 << 6),
+                             6 * (1 << 24) + 0xFFFFF0))):
         e = int(np.frexp(np.float32(value))[1])
-        quantum = float(np.ldexp(1.0, e - 31))
+        quantum = float(np.ldexp(1.0, e - 27))
         for m in ms:
             comp = float(np.float32(m * 2.0 ** -24 * quantum))
             assert comp == m * 2.0 ** -24 * quantum  # exactly representable
             assert L.orc_quantize_comp_mean(comp, value) == comp, (value, m)
     for value in (0.5, 0.75, 0.02, 1.7, 3.0e-3):
         e = int(np.frexp(np.float32(value))[1])
-        quantum = float(np.ldexp(1.0, e - 31))
-        for frac in (0.3, -0.3, 0.71, -1.5, 12.125):
+        quantum = float(np.ldexp(1.0, e - 27))
+        for frac in (0.3, -0.3, 0.71, -1.5, 5.125):
             comp = float(np.float32(frac * quantum))
             mean = np.mean([q(comp, value, k) for k in keys]) / quantum
             assert abs(mean - comp / quantum) < 5e-3, (value, frac, mean)
             assert q(comp, value, 0) == np.floor(comp / quantum) * quantum
-        for n in (-128, -77, -1, 0, 1, 64, 127):
+        for n in (-8, -5, -1, 0, 1, 4, 7):
             comp = n * quantum
             assert all(q(comp, value, k) == comp for k in keys[:512])
-        assert q(200 * quantum, value, keys[7]) == 127 * quantum
-        assert q(-200 * quantum, value, keys[7]) == -128 * quantum
+        assert q(20 * quantum, value, keys[7]) == 7 * quantum
+        assert q(-20 * quantum, value, keys[7]) == -8 * quantum
 
 
 def test_mixed_form_close_to_fp64(setup_state, params3d):
