@@ -103,9 +103,10 @@ def kernel_source_hash() -> str:
 
 
 def measured_traffic(workload: str) -> dict:
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (each scaled by what
-    it reads for a plain copy of known size), from the `TRAFFIC {...}` line of a summary in
-    profiles/ for this workload.  The line carries the hash of the diffusion kernels' sources it was
+    """Memory bytes per launch measured with rocprofv3: reads as the L2's read requests to the
+    fabric by size (TCC_EA0_RDREQ_32B/64B/128B), writes as WRITE_SIZE, both checked on a plain
+    copy of known size, from the `TRAFFIC {...}` line of a summary in profiles/ for this
+    workload.  The line carries the hash of the diffusion kernels' sources it was
     measured with; `stale` is set when the sources have changed since."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
@@ -431,7 +432,7 @@ def main() -> int:
                          "steps (8 forward runs each); empty for none")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the known-size "
-                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
+                         "kernel that checks the rocprofv3 byte counters)")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="only print the per-kernel mean of each counter in the rocprofv3 "
                          "counter-collection files under DIR")

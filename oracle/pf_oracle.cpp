@@ -438,11 +438,23 @@ template <class F> struct Domain : DomainBase {
             long nb[6];
             neighbours(i, j, k, nb);
             A del = A(0.0);
-            for (int f = 0; f < 2 * p.dims; ++f)
-              for (int q = 1; q <= 3; ++q)
-                del += sqrt(A(vph[nb[f] * 3 + q - 1]) * A(vph[c * 3 + q - 1])) *
-                       (A(vc1[s][nb[f] * 3 + q - 1]) - A(vc1[s][c * 3 + q - 1])) * dif[s][q];
-            del *= r;
+            if (F::kahan) {
+              // Mixed form (no reference order to keep): per phase the face sum of
+              // w * (c_nb - c), then the sum over phases of that times D_q * r, rounded once.
+              // A bulk cell gives (sum_f (c_f - c)) * (D_p * r), the bulk kernels' form.
+              A sq[4] = {A(0.0), A(0.0), A(0.0), A(0.0)};
+              for (int f = 0; f < 2 * p.dims; ++f)
+                for (int q = 1; q <= 3; ++q)
+                  sq[q] += sqrt(A(vph[nb[f] * 3 + q - 1]) * A(vph[c * 3 + q - 1])) *
+                           (A(vc1[s][nb[f] * 3 + q - 1]) - A(vc1[s][c * 3 + q - 1]));
+              for (int q = 1; q <= 3; ++q) del += sq[q] * (dif[s][q] * r);
+            } else {
+              for (int f = 0; f < 2 * p.dims; ++f)
+                for (int q = 1; q <= 3; ++q)
+                  del += sqrt(A(vph[nb[f] * 3 + q - 1]) * A(vph[c * 3 + q - 1])) *
+                         (A(vc1[s][nb[f] * 3 + q - 1]) - A(vc1[s][c * 3 + q - 1])) * dif[s][q];
+              del *= r;
+            }
             if (F::kahan) {
               const S y = S(del) - comp[s][c];
               const S t = conc[s][c] + y;

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
     }
     __syncthreads();
     if (interior) {
-      A del = A(0);
+      DelAcc<F, P> acc;
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         if (f < NF) {
@@ -157,23 +157,23 @@ __global__ __launch_bounds__(64 * (TY + 3)) void diffuse_kernel(
           for (int p = 0; p < P; ++p) {
             const A np = lp[buf][p][yy][xx];
             const A nc = lc[buf][p][yy][xx];
-            del += face_weight<A>(np * cur.ph[p]) * (nc - cur.c[p]) * dc.D[p];
+            acc.add(p, face_weight<A>(np * cur.ph[p]) * (nc - cur.c[p]), dc);
           }
         }
       }
       if (NF == 6) {
         A wpz[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) del += wmz[p] * (prv.c[p] - cur.c[p]) * dc.D[p];
+        for (int p = 0; p < P; ++p) acc.add(p, wmz[p] * (prv.c[p] - cur.c[p]), dc);
 #pragma unroll
         for (int p = 0; p < P; ++p) {
           wpz[p] = face_weight<A>(nxt.ph[p] * cur.ph[p]);
-          del += wpz[p] * (nxt.c[p] - cur.c[p]) * dc.D[p];
+          acc.add(p, wpz[p] * (nxt.c[p] - cur.c[p]), dc);
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) wmz[p] = wpz[p];
       }
-      del *= dc.r;
+      const A del = acc.result(dc);
       conc_update<F>(cin, cout, col + plane * z, plane * nz, ccur, del, dc.key);
     }
     prv = cur;
@@ -456,17 +456,17 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_kernel(
           for (int q = 1; q < P; ++q)
             if (fl[u] == q + 1) Dp = dc.D[q];
           const A c = A(conc_hi(cc[u]));
-          A del = A(0);
+          BulkAcc<F> acc;
 #pragma unroll
           for (int f = 0; f < 4; ++f)
-            if (f < NF) del += (xy[u][f] - c) * Dp;
+            if (f < NF) acc.add(xy[u][f] - c, Dp);
           if (NF == 6) {
             const A m = u == 0 ? zlo : A(conc_hi(cc[u > 0 ? u - 1 : 0]));
             const A p = u + 1 < nv ? A(conc_hi(cc[u + 1 < U ? u + 1 : u])) : zhi;
-            del += (m - c) * Dp;
-            del += (p - c) * Dp;
+            acc.add(m - c, Dp);
+            acc.add(p - c, Dp);
           }
-          del *= dc.r;
+          const A del = acc.result(dc.r, Dp * dc.r);
           conc_update<F>(cin, cout, plane * (z + u) + oc, plane * nz, cc[u], del, dc.key);
         } else {
           general |= 1u << u;
@@ -629,20 +629,20 @@ __global__ __launch_bounds__(64 * BY) void diffuse_bulk_vec_kernel(
           const A c = cc[u].v[v];
           const A l = v == 0 ? xm[u] : cc[u].v[v > 0 ? v - 1 : 0];
           const A r = v == V - 1 ? xp[u] : cc[u].v[v + 1 < V ? v + 1 : v];
-          A d = A(0);
-          d += (l - c) * Dp;
-          d += (r - c) * Dp;
+          BulkAcc<F> d;
+          d.add(l - c, Dp);
+          d.add(r - c, Dp);
           if (NF >= 4) {
-            d += (ym[u].v[v] - c) * Dp;
-            d += (yp[u].v[v] - c) * Dp;
+            d.add(ym[u].v[v] - c, Dp);
+            d.add(yp[u].v[v] - c, Dp);
           }
           if (NF == 6) {
             const A m = u == 0 ? zlo.v[v] : cc[u > 0 ? u - 1 : 0].v[v];
             const A p = u + 1 < nv ? cc[u + 1 < U ? u + 1 : u].v[v] : zhi.v[v];
-            d += (m - c) * Dp;
-            d += (p - c) * Dp;
+            d.add(m - c, Dp);
+            d.add(p - c, Dp);
           }
-          del[v] = d * dc.r;
+          del[v] = d.result(dc.r, Dp * dc.r);
         }
         vec_update<F, V>(cin, cout, plane * (z + u) + oc, ncell, cc[u], del, qc[u], dc.key);
       }
@@ -719,6 +719,7 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
 #pragma unroll
   for (int q = 1; q < P; ++q)
     if (pd == q + 1) Dp = dc.D[q];
+  const A Dpr = Dp * dc.r;  // the mixed form's factored D_p * r
   // per row r of the lane: tile row tr = wv*R + r, domain row y (rows past the domain load a
   // valid row and store nothing)
   int oc[R], oym[R], oyp[R], oxm[R], oxp[R];
@@ -821,18 +822,18 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
           const A c = pc.c[r].v[v];
           const A l = v == 0 ? xm[r] : pc.c[r].v[v > 0 ? v - 1 : 0];
           const A rr = v == V - 1 ? xp[r] : pc.c[r].v[v + 1 < V ? v + 1 : v];
-          A d = A(0);
-          d += (l - c) * Dp;
-          d += (rr - c) * Dp;
+          BulkAcc<F> d;
+          d.add(l - c, Dp);
+          d.add(rr - c, Dp);
           if (NF >= 4) {
-            d += (ym[r].v[v] - c) * Dp;
-            d += (yp[r].v[v] - c) * Dp;
+            d.add(ym[r].v[v] - c, Dp);
+            d.add(yp[r].v[v] - c, Dp);
           }
           if (NF == 6) {
-            d += (pm.c[r].v[v] - c) * Dp;
-            d += (pp.c[r].v[v] - c) * Dp;
+            d.add(pm.c[r].v[v] - c, Dp);
+            d.add(pp.c[r].v[v] - c, Dp);
           }
-          del[v] = d * dc.r;
+          del[v] = d.result(dc.r, Dpr);
         }
         vec_update<F, V>(cin, cout, plane * z + oc[r], ncell, pc.c[r], del, qc, dc.key);
       }
@@ -888,11 +889,10 @@ __global__ __launch_bounds__(256) void diffuse_linear_kernel(
     A nv[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) nv[f] = A(conc_hi(cin[nbi[f]]));
-    A del = A(0);
+    BulkAcc<F> acc;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) del += (nv[f] - c) * Dp;
-    del *= dc.r;
-    conc_update<F>(cin, cout, i, plane * nz, cv, del, dc.key);
+    for (int f = 0; f < NF; ++f) acc.add(nv[f] - c, Dp);
+    conc_update<F>(cin, cout, i, plane * nz, cv, acc.result(dc.r, Dp * dc.r), dc.key);
   } else if (!SPLIT) {
     full_update<F, P, NF>(cin, cout, pfrac, i, plane * nz, nbi, dc);
   }
@@ -1003,17 +1003,16 @@ __global__ __launch_bounds__(256) void diffuse_retable_kernel(
   stencil_load<F, P, NF>(cin, pfrac, i, ncell, nbi, s);
   CellPF<A> me;
   partition_cell<F, P>(s.me, copied[i] ? dc_prev : dc, me);
-  A del = A(0);
+  DelAcc<F, P> acc;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     CellPF<A> nb;
     partition_cell<F, P>(s.nb[f], copied[nbi[f]] ? dc_prev : dc, nb);
 #pragma unroll
     for (int p = 0; p < P; ++p)
-      del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
+      acc.add(p, face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]), dc);
   }
-  del *= dc.r;
-  conc_store<F>(cout, i, ncell, s.me.c, del, s.q, dc.key);
+  conc_store<F>(cout, i, ncell, s.me.c, acc.result(dc), s.q, dc.key);
 }
 
 // ---- list of the cells that need the full formula ---------------------------------------------
@@ -1068,6 +1067,10 @@ __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__res
   if (lane == 0) wsum[wave] = n;
   __syncthreads();
   if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(8) void zero_queue_kernel(int *__restrict__ queue) {
+  queue[32 * threadIdx.x] = 0;
 }
 
 // exclusive prefix sum of bcount[0..nb) in place; bcount[nb] = total.  One workgroup of 1024.

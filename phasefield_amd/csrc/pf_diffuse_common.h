@@ -116,6 +116,51 @@ __device__ inline void stencil_load(const typename F::C *__restrict__ cin,
   s.q = 0;
   if constexpr (F::kMixed) s.q = comp_load(cin, n, i);
 }
+// A cell's diffusion increment, accumulated face by face (solver.cpp:352-366).  FP64 keeps the
+// reference's order: del = sum over faces and phases of w * (c_nb - c) * D_p, then del *= r.  The
+// mixed form sums w * (c_nb - c) per phase over the faces and adds those sums times D_p * r (one
+// product, rounded once), a multiply per face and phase fewer.  For a bulk cell of phase p the
+// other phases add 0 * x = +-0, so the mixed full formula gives (sum_f (c_f - c)) * (D_p * r)
+// bitwise, the bulk kernels' form (BulkAcc), as the reference's two forms agree in FP64.
+template <class F, int P> struct DelAcc {
+  using A = typename F::A;
+  static constexpr int NS = F::kMixed ? P : 1;
+  A s[NS];
+  __device__ DelAcc() {
+#pragma unroll
+    for (int p = 0; p < NS; ++p) s[p] = A(0);
+  }
+  // the term of phase p on one face: w * (c_nb - c)
+  __device__ void add(int p, A term, const DiffConsts<A> &dc) {
+    if constexpr (F::kMixed) s[p] += term;
+    else s[0] += term * dc.D[p];
+  }
+  __device__ A result(const DiffConsts<A> &dc) const {
+    if constexpr (F::kMixed) {
+      A del = A(0);
+#pragma unroll
+      for (int p = 0; p < P; ++p) del += s[p] * (dc.D[p] * dc.r);
+      return del;
+    } else {
+      return s[0] * dc.r;
+    }
+  }
+};
+// The bulk-cell form, one phase's D_p for every face: FP64 sum_f (c_f - c) * D_p, then * r; mixed
+// (sum_f (c_f - c)) * Dpr with Dpr = D_p * r.
+template <class F> struct BulkAcc {
+  using A = typename F::A;
+  A s = A(0);
+  __device__ void add(A diff, A Dp) {
+    if constexpr (F::kMixed) s += diff;
+    else s += diff * Dp;
+  }
+  __device__ A result(A r, A Dpr) const {
+    if constexpr (F::kMixed) return s * Dpr;
+    else return s * r;
+  }
+};
+
 // partition (solver.cpp:389-405), weighted flux (solver.cpp:352-365): the cell's increment
 template <class F, int P, int NF>
 __device__ inline typename F::A stencil_del(const StencilLoad<F, P, NF> &s,
@@ -123,16 +168,16 @@ __device__ inline typename F::A stencil_del(const StencilLoad<F, P, NF> &s,
   using A = typename F::A;
   CellPF<A> me;
   partition_cell<F, P>(s.me, dc, me);
-  A del = A(0);
+  DelAcc<F, P> acc;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     CellPF<A> nb;
     partition_cell<F, P>(s.nb[f], dc, nb);
 #pragma unroll
     for (int p = 0; p < P; ++p)
-      del += face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]) * dc.D[p];
+      acc.add(p, face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]), dc);
   }
-  return del * dc.r;
+  return acc.result(dc);
 }
 // ... and the update
 template <class F, int P, int NF>

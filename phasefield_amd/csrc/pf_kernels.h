@@ -16,7 +16,9 @@ __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const typename F::C *__restrict__ conc0,
                                 const PFConsts<typename F::A> cs, const int nx, const int ny,
                                 const int nz, const long ncell, const int *__restrict__ list,
-                                const int *__restrict__ count);
+                                const int *__restrict__ count, int *__restrict__ queue);
+// the LIST walk's per-XCD counters (queue[32 * xcd], xcd < 8) set to zero: one workgroup of 8
+__global__ void zero_queue_kernel(int *__restrict__ queue);
 
 // copy cells of the sweep written directly, the others flagged 0 for the listed sweep
 template <class F, int K, int NF, bool NOFLUX>

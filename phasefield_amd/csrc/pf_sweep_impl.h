@@ -44,7 +44,8 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
     typename F::T *__restrict__ pfrac, const typename F::C *__restrict__ conc0,
     const PFConsts<typename F::A> cs, const int nx, const int ny, const int nz,
-    const long ncell, const int *__restrict__ list, const int *__restrict__ count) {
+    const long ncell, const int *__restrict__ list, const int *__restrict__ count,
+    int *__restrict__ queue) {
   using T = typename F::T;
   using A = typename F::A;
   __shared__ uint16_t sLg[kStencilMax][kSweepBS];
@@ -268,6 +269,19 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     const int n = *count;
     if (gridDim.x < 8) {
       for (int k = blockIdx.x * kSweepBS + t; k < n; k += gridDim.x * kSweepBS) body(list[k]);
+    } else if (queue) {
+      // dynamic: each wavefront takes the next 64 entries of its XCD's eighth from that XCD's
+      // counter (queue[32 * xcd], zeroed before the launch), so the entries in process stay one
+      // compact front however unevenly the cells' work is spread
+      const int xcd = int(blockIdx.x & 7u), lane = t & 63;
+      const int lo = int((long(n) * xcd) / 8), hi = int((long(n) * (xcd + 1)) / 8);
+      for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&queue[32 * xcd], 64);
+        base = __shfl(base, 0);
+        if (lo + base >= hi) break;  // wave-uniform; the counter only grows, so every wave ends
+        if (lo + base + lane < hi) body(list[lo + base + lane]);
+      }
     } else {
       const int xcd = int(blockIdx.x & 7u), rank = int(blockIdx.x >> 3);
       const int nb_x = (int(gridDim.x) - xcd + 7) / 8;  // workgroups on this XCD (>= 1)
@@ -360,7 +374,7 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
 #define PF_INST_SWEEP(F, K, P, NF, NOF, ISO, LIST)                              \
   template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO, LIST>(        \
       const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *, \
-      const PFConsts<F::A>, int, int, int, long, const int *, const int *);
+      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *);
 #define PF_INST_SWEEP_ALL(F, K, P, ISO)          \
   PF_INST_SWEEP(F, K, P, 4, false, ISO, false)   \
   PF_INST_SWEEP(F, K, P, 6, false, ISO, false)   \

@@ -138,6 +138,21 @@ def test_sweep_list_orders_bitwise(tmp_path, monkeypatch, shape, rows, form):
 
 
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("queue", ["1", "0"])
+@pytest.mark.parametrize("shape", [(70, 20, 18), (64, 23, 18)])
+def test_sweep_queue_walk_bitwise(tmp_path, monkeypatch, shape, queue, form):
+    """The listed sweep's two walks: wavefronts taking entries from per-XCD counters (the default,
+    PFAMD_SWEEP_QUEUE=1) and a fixed stride (0).  Every cell is updated exactly once either way."""
+    banner(f"sweep walk queue={queue} shape={shape} form={form}")
+    monkeypatch.setenv("PFAMD_SWEEP_QUEUE", queue)
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=6, nuclei=4)
+    gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p,
+                       lambda g, q: g.step(3), lambda o, q: o.step(3))
+    assert_same(gs, os_, 4, "sweep queue walk")
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
 @pytest.mark.parametrize("kernel,rows,split,vec,march,zchunk", [
     ("bulk", "-1", "1", "1", "1", "0"),
     ("bulk", "8", "1", "1", "1", "5"), ("bulk", "16", "1", "1", "1", "3"),
