@@ -308,7 +308,16 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
     const int history, const double cutoff, const int nx, const int ny, const int nz,
     const long ncell) {
   using T = typename F::T;
-  const long cell = long(blockIdx.x) * 256 + threadIdx.x;
+  // Workgroup b runs on XCD b % 8.  With the grid a multiple of 128 workgroups, XCD j takes the
+  // groups of 16 consecutive workgroups (4096 cells: 8 rows at nx = 512) numbered j, j + 8, ...:
+  // a cell's -y/+y neighbour rows are then in its own XCD's L2 (in plain order they belong to
+  // other XCDs), and so are its -z/+z rows, one eighth of a plane away on that XCD.
+  long blk = blockIdx.x;
+  if ((gridDim.x & 127u) == 0) {
+    const long k = blockIdx.x >> 3;
+    blk = (((k >> 4) * 8 + (blockIdx.x & 7u)) << 4) | (k & 15);
+  }
+  const long cell = blk * 256 + threadIdx.x;
   if (cell >= ncell) return;
   uint32_t g0[K];
   T v0[K];

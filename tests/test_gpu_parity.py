@@ -138,6 +138,21 @@ def test_sweep_list_orders_bitwise(tmp_path, monkeypatch, shape, rows, form):
 
 
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("shape,bc", [((64, 64, 16), 0), ((128, 32, 8), 0), ((64, 32, 32), 1)])
+def test_sweep_classify_xcd_order_bitwise(tmp_path, shape, bc, form):
+    """Shapes whose copy-cell classification runs a multiple of 128 workgroups, the case in which
+    each XCD takes interleaved groups of 16 workgroups instead of the plain order (N / 256 =
+    256, 128 and 256 here; periodic and no-flux z)."""
+    banner(f"sweep classify XCD order shape={shape} bc={bc} form={form}")
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=6, nuclei=4,
+                            bc_z="noflux" if bc else "periodic")
+    gs, os_ = run_pair(shape, 4, 2, 1, bc, form, st, p,
+                       lambda g, q: g.step(3), lambda o, q: o.step(3))
+    assert_same(gs, os_, 4, "sweep classify XCD order")
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
 @pytest.mark.parametrize("queue", ["1", "0"])
 @pytest.mark.parametrize("shape", [(70, 20, 18), (64, 23, 18)])
 def test_sweep_queue_walk_bitwise(tmp_path, monkeypatch, shape, queue, form):
