@@ -504,9 +504,10 @@ def main() -> int:
         bpc = march_bytes_per_cell(is_mixed)
         achieved = N * bpc / (march_us * 1e-6) / 1e9
         kname = march_kernel_name(is_mixed, d.nphase, noflux, rows)
-        measured = (traffic.get("march", {}).get(form)
-                    if traffic and not traffic.get("stale") and traffic.get("kernel", {}).get(form) == kname
-                    else None)
+        # the summary records each profiled tile shape's bytes by kernel name; the tuner's pick
+        # on this device selects one (none when it picked a shape the summary did not profile)
+        measured = (traffic.get("march", {}).get(kname)
+                    if traffic and not traffic.get("stale") else None)
         lbpc = listed_bytes_per_cell(is_mixed, d.nphase)
         sub_bytes = N * bpc + diff_listed * lbpc
         res[name] = {

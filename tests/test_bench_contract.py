@@ -36,9 +36,14 @@ def test_measured_traffic_names_its_sources():
     assert t, "profiles/ holds no traffic line for the default workload"
     assert t["stale"] == (t.get("sources") != bench.kernel_source_hash())
     if not t["stale"]:
-        m = t["march"]
-        # at least the compulsory bytes the marching kernel moves per cell (16 / 9 B at 512^3)
-        assert m["FormF64"] >= 16 * 512 ** 3 and m["FormMixed"] >= 9 * 512 ** 3
+        # bytes per launch by kernel name, for each profiled tile shape of both forms: at least
+        # nine tenths of the compulsory bytes the marching kernel moves per cell (16 / 9 B at 512^3;
+        # the x ends of a lane's run come from registers, the edge rows once per tile)
+        for rows in (8, 16):
+            for mixed, bpc in ((False, 16), (True, 9)):
+                name = bench.march_kernel_name(mixed, 2, False, rows)
+                assert name in t["march"], name
+                assert t["march"][name] >= 0.9 * bpc * 512 ** 3
 
 
 def test_pmc_summary_reads_counter_files(tmp_path):
