@@ -97,6 +97,225 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     c = A(conc_hi(conc0[cell]));
   }
 
+  // local_conc (value_conc1 of this cell before the update) and the driving force per phase
+  // pair (Driving_force, solver.cpp:374-387: only the concentration of the first phase enters)
+  auto driving_forces = [&](A (&dG)[4][4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) dG[a][b] = A(0);
+    if (!ISO) {
+      A lc[4];
+      lc[0] = A(0);
+#pragma unroll
+      for (int i = 1; i <= P; ++i) {
+        A t1 = A(0), t2 = A(0);
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          t1 += ph[j] * cs.pk[i][j + 1];
+          t2 += ph[j] * cs.pc[i][j + 1];
+        }
+        lc[i] = (c - t2) / t1;
+      }
+#pragma unroll
+      for (int a = 1; a <= P; ++a)
+#pragma unroll
+        for (int b = 1; b <= P; ++b) {
+          A df = A(0);
+          if (a != b) df += cs.ent[a][b] * (lc[a] - cs.eqc[a][b]) / cs.slope[a][b];
+          dG[a][b] = df;
+        }
+    }
+  };
+  // the entry values of the own slots: an entry's own value is the slot value at its position
+  // when that slot is above the cutoff (positions >= K hold neighbour-only entries: 0)
+  auto v1_of = [&](int e) -> T {
+    T r = T(0);
+#pragma unroll
+    for (int s = 0; s < K; ++s)
+      if (e == s && double(v0[s]) > cutoff) r = v0[s];
+    return r;
+  };
+  // the pair sum of entry a (solver.cpp:232-272): sum over b != a, in order, then Euler step,
+  // clamp and prune; Lb/V2b/V1b give entry b's id, neighbour sum and own value
+  auto entry_update = [&](int pa, A v1a, A v2a, int a, int nb_, const A (&dG)[4][4], auto Lb,
+                          auto V2b, auto V1b) -> double {
+    double acc = 0.0;
+    for (int b = 0; b < nb_; ++b) {
+      if (b == a) continue;
+      const A v1b = V1b(b), v2b = V2b(b);
+      if (ISO) {
+        const A tt = (v2a * v1b - v2b * v1a) * cs.f1 + (v1a - v1b) * cs.f2;
+        acc += double(tt);
+      } else {
+        const int pb = code_phase(Lb(b));
+        if (pa - pb == 2 || pb - pa == 2) continue;
+        A tt = (v2a * v1b - v2b * v1a) * cs.f1;
+        tt += (v1a - v1b) * cs.f2;
+        tt *= pair_pick<A, P>(cs.sig, pa, pb);
+        tt += cs.f3 * sqrt(v1a * v1b) * pair_pick<A, P>(dG, pa, pb);
+        tt *= pair_pick<A, P>(cs.M, pa, pb);
+        acc += double(tt);
+      }
+    }
+    acc *= cs.dt;
+    acc += double(v1a);
+    if (acc > 1.0)
+      acc = 1.0;
+    else if (acc < cutoff)
+      acc = 0.0;
+    return acc;
+  };
+
+  if constexpr (K == 4 && !ISO) {
+    // Registers for a union of at most four entries (almost every listed cell: two grains at a
+    // grain boundary, three at a triple line or an austenite front), the same algorithm as the
+    // general path below: own slots at their slot positions, first-match search, appends at
+    // position n, pair sums in entry order, first-maximum selection sort.  With K = 4 the own
+    // entries lie in positions 0..3 and an append reaches position 4 only as the fifth entry, so
+    // until then positions 4..9 are empty and searching them finds nothing; a cell whose union
+    // would need a fifth entry takes the general path from the start.
+    uint32_t L[4];
+    T S2[4];
+    int m = 0;
+    bool over = false;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const bool own = double(v0[e]) > cutoff;
+      L[e] = own ? g0[e] : 0u;
+      S2[e] = T(0);
+      m += own ? 1 : 0;
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      bool live = true;
+#pragma unroll
+      for (int sl = 0; sl < K; ++sl) {
+        const uint32_t g = gn[f][sl];
+        live = live && (g != 0);
+        if (live) {
+          const T v = vn[f][sl];
+          const int kk = L[0] == g ? 0 : L[1] == g ? 1 : L[2] == g ? 2 : L[3] == g ? 3 : -1;
+          if (kk >= 0) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (e == kk) S2[e] += v;
+          } else if (m < 4) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (e == m) {
+                L[e] = g;
+                S2[e] += v;
+              }
+            ++m;
+          } else {
+            over = true;
+          }
+        }
+      }
+    }
+    if (!over) {
+      if (m == 1) {
+        using V = typename IdRowT<K>::V;
+        reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
+#pragma unroll
+        for (int s = 0; s < K; ++s) phi2[cell * K + s] = v0[s];
+        return;
+      }
+      A dG[4][4];
+      driving_forces(dG);
+      // the pair sums with entry indices known at compile time (no selects into the arrays)
+      A V1[4], V2[4];
+      int PH[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        V1[e] = A(double(v0[e]) > cutoff ? v0[e] : T(0));
+        V2[e] = A(S2[e]);
+        PH[e] = code_phase(L[e]);
+      }
+      double D[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        D[a] = 0.0;
+        if (a < m) {
+          double acc = 0.0;
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            if (b == a || b >= m) continue;
+            const int pa = PH[a], pb = PH[b];
+            if (pa - pb == 2 || pb - pa == 2) continue;
+            A tt = (V2[a] * V1[b] - V2[b] * V1[a]) * cs.f1;
+            tt += (V1[a] - V1[b]) * cs.f2;
+            tt *= pair_pick<A, P>(cs.sig, pa, pb);
+            tt += cs.f3 * sqrt(V1[a] * V1[b]) * pair_pick<A, P>(dG, pa, pb);
+            tt *= pair_pick<A, P>(cs.M, pa, pb);
+            acc += double(tt);
+          }
+          acc *= cs.dt;
+          acc += double(V1[a]);
+          if (acc > 1.0)
+            acc = 1.0;
+          else if (acc < cutoff)
+            acc = 0.0;
+          D[a] = acc;
+        }
+      }
+      // Sort_xyz's first K passes over positions 0..3; positions 4..9 hold 0, which never
+      // exceeds a value that survived the clamp (0 or >= cutoff), so they are never chosen
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        int mi = a;
+        double best = D[a];
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b)
+          if (D[b] > best) {
+            best = D[b];
+            mi = b;
+          }
+        const double da = D[a];
+        const uint32_t la = L[a];
+        uint32_t lm = la;
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b)
+          if (mi == b) lm = L[b];
+#pragma unroll
+        for (int b = a + 1; b < 4; ++b)
+          if (mi == b) {
+            D[b] = da;
+            L[b] = la;
+          }
+        D[a] = best;
+        L[a] = lm;
+      }
+      double sum = 0.0, vp[4] = {0.0, 0.0, 0.0, 0.0};
+      uint32_t go[K];
+      double out[K];
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        const double d = D[s];
+        if (d > cutoff) {
+          go[s] = L[s];
+          out[s] = d;
+          const int p = code_phase(go[s]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (q == p) vp[q] += d;
+          sum += d;
+        } else {
+          go[s] = 0;
+          out[s] = 0.0;
+        }
+      }
+      const double inv = 1. / sum;
+      store_ids<K>(id2, cell, go);
+#pragma unroll
+      for (int s = 0; s < K; ++s) phi2[cell * K + s] = T(out[s] * inv);
+#pragma unroll
+      for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
+      return;
+    }
+  }
+
   for (int e = 0; e < kStencilMax; ++e) {
     sLg[e][t] = 0;
     sV2[e][t] = T(0);
@@ -109,16 +328,6 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
       sLg[s][t] = uint16_t(g0[s]);
     }
   }
-
-  // the entry values of the own slots: an entry's own value is the slot value at its position
-  // when that slot is above the cutoff (positions >= K hold neighbour-only entries: 0)
-  auto v1_of = [&](int e) -> T {
-    T r = T(0);
-#pragma unroll
-    for (int s = 0; s < K; ++s)
-      if (e == s && double(v0[s]) > cutoff) r = v0[s];
-    return r;
-  };
 
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -153,66 +362,15 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     return;
   }
 
-  // local_conc (value_conc1 of this cell before the update) and the driving force per phase
-  // pair (Driving_force, solver.cpp:374-387: only the concentration of the first phase enters)
   A dG[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) dG[a][b] = A(0);
-  if (!ISO) {
-    A lc[4];
-    lc[0] = A(0);
-#pragma unroll
-    for (int i = 1; i <= P; ++i) {
-      A t1 = A(0), t2 = A(0);
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        t1 += ph[j] * cs.pk[i][j + 1];
-        t2 += ph[j] * cs.pc[i][j + 1];
-      }
-      lc[i] = (c - t2) / t1;
-    }
-#pragma unroll
-    for (int a = 1; a <= P; ++a)
-#pragma unroll
-      for (int b = 1; b <= P; ++b) {
-        A df = A(0);
-        if (a != b) df += cs.ent[a][b] * (lc[a] - cs.eqc[a][b]) / cs.slope[a][b];
-        dG[a][b] = df;
-      }
-  }
+  driving_forces(dG);
 
   for (int a = 0; a < kStencilMax; ++a) sDel[a][t] = 0.0;
-  for (int a = 0; a < n; ++a) {
-    const int pa = code_phase(sLg[a][t]);
-    const A v1a = A(v1_of(a)), v2a = A(sV2[a][t]);
-    double acc = 0.0;
-    for (int b = 0; b < n; ++b) {
-      if (b == a) continue;
-      const A v1b = A(v1_of(b)), v2b = A(sV2[b][t]);
-      if (ISO) {
-        const A tt = (v2a * v1b - v2b * v1a) * cs.f1 + (v1a - v1b) * cs.f2;
-        acc += double(tt);
-      } else {
-        const int pb = code_phase(sLg[b][t]);
-        if (pa - pb == 2 || pb - pa == 2) continue;
-        A tt = (v2a * v1b - v2b * v1a) * cs.f1;
-        tt += (v1a - v1b) * cs.f2;
-        tt *= pair_pick<A, P>(cs.sig, pa, pb);
-        tt += cs.f3 * sqrt(v1a * v1b) * pair_pick<A, P>(dG, pa, pb);
-        tt *= pair_pick<A, P>(cs.M, pa, pb);
-        acc += double(tt);
-      }
-    }
-    acc *= cs.dt;
-    acc += double(v1a);
-    if (acc > 1.0)
-      acc = 1.0;
-    else if (acc < cutoff)
-      acc = 0.0;
-    sDel[a][t] = acc;
-  }
+  for (int a = 0; a < n; ++a)
+    sDel[a][t] = entry_update(code_phase(sLg[a][t]), A(v1_of(a)), A(sV2[a][t]), a, n, dG,
+                              [&](int b) -> uint32_t { return sLg[b][t]; },
+                              [&](int b) -> A { return A(sV2[b][t]); },
+                              [&](int b) -> A { return A(v1_of(b)); });
 
   // Sort_xyz (solver.cpp:426-432): only the first K positions are read afterwards, and the
   // first K passes of the selection sort fix them.
