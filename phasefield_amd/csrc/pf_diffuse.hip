@@ -907,6 +907,7 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
     const typename F::T *__restrict__ pfrac, const int *__restrict__ list,
     const int *__restrict__ count, const DiffConsts<typename F::A> dc, const int nx,
     const int ny, const int nz) {
+  using A = typename F::A;
   const int n = *count;
   const long plane = long(nx) * ny, ncell = plane * nz;
   auto stencil = [&](long i, long (&nbi)[6]) {
@@ -927,6 +928,87 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
     // owns the code byte of cells 2j and 2j + 1: their values and the byte are written with plain
     // stores (a cell of the pair that is not listed keeps the code the bulk kernel wrote).  The
     // loads of both cells are issued before the arithmetic.
+    if ((nx & 1) == 0) {
+      // nx even: cells 2j and 2j + 1 are x neighbours in one row, so the pair's two stencils share
+      // the pair's own cells: 12 distinct cells and partitions instead of 14, and the face
+      // between the two (its weight sqrt(phi_a phi_b) is the same product both ways) is weighed
+      // once.  Both cells are computed, which keeps the wavefront uniform; only the listed ones
+      // are stored.
+      for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += stride) {
+        const uint32_t e = uint32_t(list[k]);
+        const long j = long(e & 0x3fffffffu);
+        const uint32_t m = e >> 30;
+        const long a = 2 * j, b = a + 1;
+        // a's stencil (x even, so a + 1 = b is its +x neighbour); b's: its -x neighbour is a, its
+        // +x neighbour wraps to the row's start after the row's last cell, its y and z neighbours
+        // are a's shifted by one
+        int xa, ya, za;
+        cell_coords(a, nx, ny, xa, ya, za);
+        long na[6];
+        na[0] = xa == 0 ? a + nx - 1 : a - 1;
+        na[1] = b;
+        na[2] = a + long(nx) * (wrap(ya - 1, ny) - ya);
+        na[3] = a + long(nx) * (wrap(ya + 1, ny) - ya);
+        {
+          const int zm = NOFLUX ? (za == 0 ? za : za - 1) : wrap(za - 1, nz);
+          const int zp = NOFLUX ? (za == nz - 1 ? za : za + 1) : wrap(za + 1, nz);
+          na[4] = a + plane * (zm - za);
+          na[5] = a + plane * (zp - za);
+        }
+        const long b_xp = xa + 2 < nx ? b + 1 : b + 1 - nx;
+        const long idx[12] = {a,         b,         na[0],     b_xp,      na[2],     na[3],
+                              na[4],     na[5],     na[2] + 1, na[3] + 1, na[4] + 1, na[5] + 1};
+        RawCell<F, P> r[12];
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+          r[q].c = cin[idx[q]];
+#pragma unroll
+          for (int p = 0; p < P; ++p) r[q].ph[p] = pfrac[idx[q] * P + p];
+        }
+        uint8_t *cb = comp_bytes(cout, ncell) + j;
+        const uint32_t cin_byte = comp_bytes(cin, ncell)[j];
+        const uint32_t old = m == 3u ? 0u : uint32_t(*cb);
+        CellPF<A> pc[12];
+#pragma unroll
+        for (int q = 0; q < 12; ++q) partition_cell<F, P>(r[q], dc, pc[q]);
+        A wab[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) wab[p] = face_weight<A>(pc[1].ph[p] * pc[0].ph[p]);
+        // faces in the reference's order (-x, +x, -y, +y, -z, +z)
+        const int fa[6] = {2, 1, 4, 5, 6, 7}, fb[6] = {0, 3, 8, 9, 10, 11};
+        DelAcc<F, P> da, db;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const CellPF<A> &x = pc[fa[f]];
+            const A w = f == 1 ? wab[p] : face_weight<A>(x.ph[p] * pc[0].ph[p]);
+            da.add(p, w * (x.c[p] - pc[0].c[p]), dc);
+          }
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            const CellPF<A> &x = pc[fb[f]];
+            const A w = f == 0 ? wab[p] : face_weight<A>(x.ph[p] * pc[1].ph[p]);
+            db.add(p, w * (x.c[p] - pc[1].c[p]), dc);
+          }
+        CompCode qa, qb;
+        const float ta = mixed_step(r[0].c, da.result(dc), comp_unpack(cin_byte), comp_hash(uint32_t(a), dc.key), qa);
+        const float tb = mixed_step(r[1].c, db.result(dc), comp_unpack(cin_byte >> 4), comp_hash(uint32_t(b), dc.key), qb);
+        uint32_t byte = old;
+        if (m & 1u) {
+          cout[a] = ta;
+          byte = (byte & 0xf0u) | (uint32_t(qa) & 15u);
+        }
+        if (m & 2u) {
+          cout[b] = tb;
+          byte = (byte & 0x0fu) | ((uint32_t(qb) & 15u) << 4);
+        }
+        *cb = uint8_t(byte);
+      }
+      return;
+    }
     for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += stride) {
       const uint32_t e = uint32_t(list[k]);
       const long j = long(e & 0x3fffffffu);

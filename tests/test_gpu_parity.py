@@ -137,6 +137,19 @@ def test_sweep_list_orders_bitwise(tmp_path, monkeypatch, shape, rows, form):
     assert_same(gs, os_, 4, f"sweep list rows={rows}")
 
 
+@pytest.mark.parametrize("shape,bc", [((33, 20, 12), 0), ((35, 18, 1), 0), ((41, 16, 10), 1)])
+def test_mixed_listed_pass_odd_nx_bitwise(tmp_path, shape, bc):
+    """The mixed listed pass with nx odd, where the two cells of a code byte can lie in different
+    rows (no shared stencil): each listed cell of a byte pair gets its own stencil."""
+    banner(f"mixed listed pass odd nx shape={shape} bc={bc}")
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=5, nuclei=4,
+                            bc_z="noflux" if bc else "periodic")
+    gs, os_ = run_pair(shape, 4, 2, 1, bc, oracle.ORC_MIXED, st, p,
+                       lambda g, q: g.step(2), lambda o, q: o.step(2))
+    assert_same(gs, os_, 4, "mixed listed pass, odd nx")
+
+
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
 @pytest.mark.parametrize("shape,bc", [((64, 64, 16), 0), ((128, 32, 8), 0), ((64, 32, 32), 1)])
 def test_sweep_classify_xcd_order_bitwise(tmp_path, shape, bc, form):
