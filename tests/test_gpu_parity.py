@@ -151,11 +151,13 @@ def test_mixed_listed_pass_odd_nx_bitwise(tmp_path, shape, bc):
 
 
 @pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
-@pytest.mark.parametrize("shape,bc", [((64, 64, 16), 0), ((128, 32, 8), 0), ((64, 32, 32), 1)])
+@pytest.mark.parametrize("shape,bc", [((64, 64, 16), 0), ((128, 32, 8), 0), ((64, 32, 32), 1),
+                                      ((256, 16, 8), 0), ((256, 12, 6), 1)])
 def test_sweep_classify_xcd_order_bitwise(tmp_path, shape, bc, form):
     """Shapes whose copy-cell classification runs a multiple of 128 workgroups, the case in which
     each XCD takes interleaved groups of 16 workgroups instead of the plain order (N / 256 =
-    256, 128 and 256 here; periodic and no-flux z)."""
+    256, 128, 256 and 128 here; periodic and no-flux z), and rows of 256 cells, whose workgroups
+    take their y and z from the workgroup (256 x 12 x 6: 72 workgroups, plain order)."""
     banner(f"sweep classify XCD order shape={shape} bc={bc} form={form}")
     nx, ny, nz = shape
     m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=6, nuclei=4,
