@@ -736,7 +736,7 @@ template <int K, int P, int NF, bool NOF> struct Adj {
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, st, ablock, int(nl),
                            static_cast<int *>(nullptr));
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, st, sd.flag, keep, N,
-                           ablock, alist, ListOrder{}, static_cast<int *>(nullptr));
+                           ablock, alist, ListOrder{});
       }
       const int mtx = (v.nx + 127) / 128, mty = mby ? (v.ny + mby - 1) / mby : 0;
       const long mnb = mby ? long(mtx) * mty * ((v.nz + mzc - 1) / mzc) : 0;

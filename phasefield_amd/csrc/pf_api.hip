@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -76,6 +77,8 @@ struct pf_handle {
   int march_by = 0;          // its tile rows and z-chunk, picked by timing at first use
   int march_zc = 0;
   int *list_len = nullptr;   // [0]: cells the last sweep listed, [1]: cells the diffusion lists
+  int *ccount = nullptr;     // pair lists: listed cells per list block ([nblocks + 1])
+  bool ccount_pending = false;  // ccount not yet summed into list_len[1]
   bool time_march = false;   // time the next plane-marching launch (timing kind 2)
   int *glist = nullptr;      // indices of the listed cells (sweep: row tiles, diffusion: storage)
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
@@ -251,14 +254,24 @@ struct Timer {
   }
 };
 
-// Order of the sweep's list: row tiles of about 4096 cells per tile-plane (nx = 512: 8 rows), so
+// Order of the sweep's list: row tiles of about 32768 cells per tile-plane (nx = 512: 64 rows), so
 // that the -z/+z neighbour rows of the cells a walk is on were touched a tile-plane earlier and are
-// still in L2; storage order for one plane (2D) or when PFAMD_SWEEP_ROWS=0.
+// still in L2, and a tile's -y/+y edge rows, read again by the next tile, are few; storage order for
+// one plane (2D) or when PFAMD_SWEEP_ROWS=0.  Measured at 512^3 with the per-XCD walk (read bytes
+// per listed cell, mixed / FP64): 8 rows 83 / 118, 32 rows 76 / 108, 64 rows 75 / 111,
+// 128 rows 90 / 179, storage order 150 / 211.
 inline ListOrder sweep_list_order(const pf_handle *h) {
   int rows = h->sweep_rows;
-  if (rows < 0) rows = std::max(1, 4096 / std::max(h->d.nx, 1));
+  if (rows < 0) rows = std::max(1, 32768 / std::max(h->d.nx, 1));
   if (h->d.nz == 1) rows = 0;
   return ListOrder{h->d.nx, h->d.ny, h->d.nz, std::min(rows, h->d.ny)};
+}
+
+// the largest float not above the cutoff: for a float v, v > it exactly when double(v) > cutoff
+inline float cutoff_below(double cutoff) {
+  float f = float(cutoff);
+  if (double(f) > cutoff) f = std::nextafter(f, -std::numeric_limits<float>::infinity());
+  return f;
 }
 
 template <class F, int K, int P, int NF, bool NOF, bool ISO>
@@ -276,7 +289,7 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     const unsigned nb = blocks_for(h->N), nl = list_blocks(h->N);
     hipLaunchKernelGGL((sweep_classify_kernel<F, K, NF, NOF>), dim3(nb), dim3(256), 0, h->stream,
                        h->id[a], phi1, h->id[b], phi2, h->sflag, h->sweep_hist ? 1 : 0,
-                       h->p.cutoff, h->d.nx, h->d.ny, h->d.nz, h->N);
+                       h->p.cutoff, cutoff_below(h->p.cutoff), h->d.nx, h->d.ny, h->d.nz, h->N);
     const ListOrder order = sweep_list_order(h);
     hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
                        static_cast<const unsigned *>(nullptr), h->N, h->gblock, order,
@@ -284,8 +297,7 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock, int(nl),
                        h->list_len);
     hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->sflag,
-                       static_cast<const unsigned *>(nullptr), h->N, h->gblock, h->glist, order,
-                       static_cast<int *>(nullptr));
+                       static_cast<const unsigned *>(nullptr), h->N, h->gblock, h->glist, order);
     // Every workgroup resident at once (the device's CUs times the kernel's occupancy), each
     // XCD's wavefronts taking the next entries of its eighth of the list from a counter: the
     // entries in process stay one compact front (512^3: 190 -> 83 B read per listed cell, mixed).
@@ -493,17 +505,19 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
         // bulk kernels leave the flag-0 cells
         const unsigned *keep = march ? h->bulk_count : nullptr;
         const unsigned nl = list_blocks(h->N);
-        // mixed: entries are byte pairs of cells (their codes share a byte); list_len[1] counts
-        // the listed cells either way
+        // mixed: entries are byte pairs of cells (their codes share a byte); the count kernel
+        // keeps per-block counts of the listed cells, summed into list_len[1] when they are asked
+        // for (pf_list_lengths); FP64: list_len[1] is the list's length
         ListOrder order;
         order.pairs = F::kMixed;
-        int *cells = F::kMixed ? h->list_len + 1 : nullptr;
+        int *cells = F::kMixed ? h->ccount : nullptr;
         hipLaunchKernelGGL(general_count_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag, keep,
                            h->N, h->gblock, order, cells);
         hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->gblock,
                            int(nl), F::kMixed ? static_cast<int *>(nullptr) : h->list_len + 1);
         hipLaunchKernelGGL(general_scatter_kernel, dim3(nl), dim3(256), 0, h->stream, h->flag,
-                           keep, h->N, h->gblock, h->glist, order, cells);
+                           keep, h->N, h->gblock, h->glist, order);
+        h->ccount_pending = F::kMixed;
       }
       h->flag_valid = true;
       h->flag_nf = NF;
@@ -657,6 +671,7 @@ void free_all(pf_handle *h) {
   if (h->bulk_count) hipFree(h->bulk_count);
   if (h->glist) hipFree(h->glist);
   if (h->list_len) hipFree(h->list_len);
+  if (h->ccount) hipFree(h->ccount);
   if (h->sflag) hipFree(h->sflag);
   if (h->squeue) hipFree(h->squeue);
   if (h->gblock) hipFree(h->gblock);
@@ -763,6 +778,9 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     return rc;
   if ((rc = chk(hipMalloc(&h->glist, N * 4), "alloc full-formula list")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->list_len, 2 * sizeof(int)), "alloc list lengths")) != PF_OK) return rc;
+  if ((rc = chk(hipMalloc(&h->ccount, (list_blocks(h->N) + 1) * sizeof(int)), "alloc list counts")) !=
+      PF_OK)
+    return rc;
   if ((rc = chk(hipMemsetAsync(h->list_len, 0, 2 * sizeof(int), h->stream), "clear")) != PF_OK)
     return rc;
   if ((rc = chk(hipMalloc(&h->sflag, N), "alloc sweep copy flags")) != PF_OK) return rc;
@@ -1187,6 +1205,12 @@ int pf_list_lengths(pf_handle *h, int64_t *sweep_listed, int64_t *diffusion_list
   if (!h) return PF_ERR_INVALID;
   hipSetDevice(h->d.device);
   int v[2] = {0, 0};
+  if (h->ccount_pending) {  // the pair list's per-block cell counts, summed once
+    hipLaunchKernelGGL(general_scan_kernel, dim3(1), dim3(1024), 0, h->stream, h->ccount,
+                       int(list_blocks(h->N)), h->list_len + 1);
+    HIPCHK(h, hipGetLastError());
+    h->ccount_pending = false;
+  }
   HIPCHK(h, hipMemcpyAsync(v, h->list_len, sizeof(v), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   if (sweep_listed) *sweep_listed = v[0];

@@ -305,6 +305,13 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict
   }
 }
 
+// 0x80 in each byte of x that is 0, 0 in the others: the low seven bits of a byte plus 0x7f set its
+// top bit unless they are all 0, and no sum carries into the next byte
+__device__ inline uint32_t zero_bytes(uint32_t x) {
+  const uint32_t t = (x & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+  return ~(t | x | 0x7f7f7f7fu);
+}
+
 // The same for nx % 4 == 0, four x-consecutive cells per thread: their own bytes, their -y/+y and
 // -z/+z neighbours are one aligned 32-bit word each, and only the two x ends are single bytes.
 template <int NF, bool NOFLUX>
@@ -315,40 +322,42 @@ __global__ __launch_bounds__(256) void classify4_kernel(const uint8_t *__restric
                                                         unsigned *__restrict__ count) {
   const uint32_t *__restrict__ u32 = reinterpret_cast<const uint32_t *>(unit);
   unsigned n1 = 0, n2 = 0, n3 = 0;  // wave-uniform
-  const long nq = ncell / 4, stride = long(gridDim.x) * 256, plane = long(nx) * ny;
+  const long nq = ncell / 4, stride = long(gridDim.x) * 256;
+  // word offsets of the neighbours (32-bit: a domain holds fewer than 2^31 cells)
+  const uint32_t qx = uint32_t(nx) / 4, qplane = qx * uint32_t(ny), qvol = qplane * uint32_t(nz);
   for (long base = long(blockIdx.x) * 256; base < nq; base += stride) {
     const long q = base + threadIdx.x;
-    int p[4] = {0, 0, 0, 0};
+    uint32_t out = 0;
     if (q < nq) {
-      const long c = q * 4;
+      const uint32_t q32 = uint32_t(q), c = 4 * q32;
       int x, y, z;
       cell_coords(c, nx, ny, x, y, z);
-      const uint32_t me = u32[q];
+      const uint32_t me = u32[q32];
       if (me != 0u) {
-        const uint32_t ym = NF >= 4 ? u32[(c + long(nx) * (wrap(y - 1, ny) - y)) / 4] : me;
-        const uint32_t yp = NF >= 4 ? u32[(c + long(nx) * (wrap(y + 1, ny) - y)) / 4] : me;
-        uint32_t zm = me, zp = me;
+        uint32_t ym = me, yp = me, zm = me, zp = me;
+        if (NF >= 4) {
+          ym = u32[y == 0 ? q32 + qplane - qx : q32 - qx];
+          yp = u32[y == ny - 1 ? q32 + qx - qplane : q32 + qx];
+        }
         if (NF == 6) {
-          const int zl = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
-          const int zh = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
-          zm = u32[(c + plane * (zl - z)) / 4];
-          zp = u32[(c + plane * (zh - z)) / 4];
+          zm = u32[z == 0 ? (NOFLUX ? q32 : q32 + qvol - qplane) : q32 - qplane];
+          zp = u32[z == nz - 1 ? (NOFLUX ? q32 : q32 + qplane - qvol) : q32 + qplane];
         }
-        const int xl = unit[c - x + wrap(x - 1, nx)], xr = unit[c - x + wrap(x + 4, nx)];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          int pv = int((me >> (8 * v)) & 0xffu);
-          const int l = v == 0 ? xl : int((me >> (8 * (v - 1))) & 0xffu);
-          const int r = v == 3 ? xr : int((me >> (8 * (v + 1))) & 0xffu);
-          if (l != pv || r != pv) pv = 0;
-          if (int((ym >> (8 * v)) & 0xffu) != pv || int((yp >> (8 * v)) & 0xffu) != pv) pv = 0;
-          if (int((zm >> (8 * v)) & 0xffu) != pv || int((zp >> (8 * v)) & 0xffu) != pv) pv = 0;
-          p[v] = pv;
-        }
+        const uint32_t xl = unit[x == 0 ? c + uint32_t(nx) - 1u : c - 1u];
+        const uint32_t xr = unit[x + 4 == nx ? c - uint32_t(x) : c + 4u];
+        // byte v of `left` / `right` is the -x / +x neighbour of cell v; a cell keeps its byte when
+        // all its neighbours' bytes equal it (byte-wise, no carry between bytes)
+        const uint32_t left = (me << 8) | xl, right = (me >> 8) | (xr << 24);
+        const uint32_t diff = (me ^ left) | (me ^ right) | (me ^ ym) | (me ^ yp) | (me ^ zm) |
+                              (me ^ zp);
+        const uint32_t same = zero_bytes(diff);  // 0x80 in the bytes of the cells that keep theirs
+        out = me & (same | (same - (same >> 7)));
       }
-      reinterpret_cast<uint32_t *>(flag)[q] =
-          uint32_t(p[0]) | uint32_t(p[1]) << 8 | uint32_t(p[2]) << 16 | uint32_t(p[3]) << 24;
+      reinterpret_cast<uint32_t *>(flag)[q32] = out;
     }
+    int p[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) p[v] = int((out >> (8 * v)) & 0xffu);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       n1 += unsigned(__popcll(__ballot(p[v] == 1)));
@@ -971,27 +980,41 @@ __global__ __launch_bounds__(256) void diffuse_general_kernel(
         CellPF<A> pc[12];
 #pragma unroll
         for (int q = 0; q < 12; ++q) partition_cell<F, P>(r[q], dc, pc[q]);
-        A wab[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) wab[p] = face_weight<A>(pc[1].ph[p] * pc[0].ph[p]);
-        // faces in the reference's order (-x, +x, -y, +y, -z, +z)
+        // the face weights: the shared a|b face, then a's other faces and b's other faces
+        constexpr int NW = (2 * NF - 1) * P;
+        A prod[NW], w[NW];
         const int fa[6] = {2, 1, 4, 5, 6, 7}, fb[6] = {0, 3, 8, 9, 10, 11};
+        auto wi = [](int side, int f, int p) {  // index in prod/w of face f of cell a (0) / b (1)
+          return side == 0 ? (f == 1 ? p : P + ((f == 0 ? 0 : f - 1) * P) + p)
+                           : (f == 0 ? p : P + (NF - 1) * P + (f - 1) * P + p);
+        };
+#pragma unroll
+        for (int p = 0; p < P; ++p) prod[p] = pc[1].ph[p] * pc[0].ph[p];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            if (f != 1) prod[wi(0, f, p)] = pc[fa[f]].ph[p] * pc[0].ph[p];
+            if (f != 0) prod[wi(1, f, p)] = pc[fb[f]].ph[p] * pc[1].ph[p];
+          }
+        face_weights<A>(prod, w);
+        // faces in the reference's order (-x, +x, -y, +y, -z, +z)
         DelAcc<F, P> da, db;
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const CellPF<A> &x = pc[fa[f]];
-            const A w = f == 1 ? wab[p] : face_weight<A>(x.ph[p] * pc[0].ph[p]);
-            da.add(p, w * (x.c[p] - pc[0].c[p]), dc);
+            const A term = w[wi(0, f, p)] * (pc[fa[f]].c[p] - pc[0].c[p]);
+            if (f == 0) da.first(p, term, dc);
+            else da.add(p, term, dc);
           }
 #pragma unroll
         for (int f = 0; f < NF; ++f)
 #pragma unroll
           for (int p = 0; p < P; ++p) {
-            const CellPF<A> &x = pc[fb[f]];
-            const A w = f == 0 ? wab[p] : face_weight<A>(x.ph[p] * pc[1].ph[p]);
-            db.add(p, w * (x.c[p] - pc[1].c[p]), dc);
+            const A term = w[wi(1, f, p)] * (pc[fb[f]].c[p] - pc[1].c[p]);
+            if (f == 0) db.first(p, term, dc);
+            else db.add(p, term, dc);
           }
         CompCode qa, qb;
         const float ta = mixed_step(r[0].c, da.result(dc), comp_unpack(cin_byte), comp_hash(uint32_t(a), dc.key), qa);
@@ -1140,113 +1163,151 @@ __global__ __launch_bounds__(256) void general_count_kernel(const uint8_t *__res
   __shared__ int wsum[4];
   const long first = long(blockIdx.x) * (256L * kListChunk) + long(threadIdx.x) * kListChunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ int csum[4];
   const int keep = count ? bulk_phase(count) : 0;
-  if (cells && blockIdx.x == 0 && threadIdx.x == 0) *cells = 0;  // the scatter kernel adds to it
   const unsigned bits = listed_bits(flag, first, ncell, count, keep, order);
   int n = __popc(order.pairs ? pair_bits(bits) : bits);
+  int nc = __popc(bits);
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) n += __shfl_xor(n, d);
-  if (lane == 0) wsum[wave] = n;
+  for (int d = 32; d >= 1; d >>= 1) {
+    n += __shfl_xor(n, d);
+    nc += __shfl_xor(nc, d);
+  }
+  if (lane == 0) {
+    wsum[wave] = n;
+    csum[wave] = nc;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (threadIdx.x == 0) {
+    bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (cells) cells[blockIdx.x] = csum[0] + csum[1] + csum[2] + csum[3];
+  }
 }
 
 __global__ __launch_bounds__(8) void zero_queue_kernel(int *__restrict__ queue) {
   queue[32 * threadIdx.x] = 0;
 }
 
-// exclusive prefix sum of bcount[0..nb) in place; bcount[nb] = total.  One workgroup of 1024.
+// exclusive prefix sum of bcount[0..nb) in place; bcount[nb] = total.  One workgroup of 1024, in
+// passes of 16 consecutive counts per thread (four 16-byte loads issued together, a wave scan and
+// the 16 wave totals from LDS), the running total carried from pass to pass.
 __global__ __launch_bounds__(1024) void general_scan_kernel(int *__restrict__ bcount, int nb,
                                                              int *__restrict__ total_out) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
-  const int lo = min(nb, t * per), hi = min(nb, lo + per);
-  int s = 0;
-  for (int k = lo; k < hi; ++k) s += bcount[k];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    const int v = t >= off ? part[t - off] : 0;
+  __shared__ int wsum[16];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  int carry = 0;
+  for (int base = 0; base < nb; base += 1024 * 16) {
+    const int i0 = base + t * 16;
+    const bool full = i0 + 16 <= nb;
+    int v[16];
+    if (full) {
+      const int4 *p = reinterpret_cast<const int4 *>(bcount + i0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int4 x = p[q];
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        v[4 * q + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = i0 + q < nb ? bcount[i0 + q] : 0;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sum += v[q];
+    int incl = sum;  // inclusive scan over the wave's lanes
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(incl, d);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[wv] = incl;
     __syncthreads();
-    part[t] += v;
-    __syncthreads();
+    int woff = 0, pass_total = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const int x = wsum[w];
+      woff += w < wv ? x : 0;
+      pass_total += x;
+    }
+    int run = carry + woff + incl - sum;  // exclusive start of this thread's counts
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = v[q];
+      v[q] = run;
+      run += c;
+    }
+    if (full) {
+      int4 *p = reinterpret_cast<int4 *>(bcount + i0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) p[q] = make_int4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (i0 + q < nb) bcount[i0 + q] = v[q];
+    }
+    carry += pass_total;
+    __syncthreads();  // wsum is rewritten by the next pass
   }
-  int run = part[t] - s;  // exclusive start of this thread's range
-  for (int k = lo; k < hi; ++k) {
-    const int c = bcount[k];
-    bcount[k] = run;
-    run += c;
-  }
-  if (t == 1023) {
-    bcount[nb] = part[1023];
-    if (total_out) *total_out = part[1023];  // the list length, kept for diagnostics
+  if (t == 0) {
+    bcount[nb] = carry;
+    if (total_out) *total_out = carry;  // the list length, kept for diagnostics
   }
 }
 
+// Each lane writes its entries, in order, into its wave's LDS row at the exclusive sum of the
+// counts of the lanes before it; the wave then copies the row to the list with consecutive lanes
+// on consecutive entries (whole-line stores instead of up to 16 scattered stores per lane).
 __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__restrict__ flag,
                                                               const unsigned *__restrict__ count,
                                                               long ncell,
                                                               const int *__restrict__ boff,
                                                               int *__restrict__ list,
-                                                              const ListOrder order,
-                                                              int *__restrict__ cells) {
+                                                              const ListOrder order) {
   __shared__ int wsum[4];
+  __shared__ int stage[4][64 * kListChunk];
   const long first = long(blockIdx.x) * (256L * kListChunk) + long(threadIdx.x) * kListChunk;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int keep = count ? bulk_phase(count) : 0;
   unsigned bits = listed_bits(flag, first, ncell, count, keep, order);
-  if (cells) {  // listed cells of the wave, one atomic per wave
-    int nc = __popc(bits);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) nc += __shfl_xor(nc, d);
-    if (lane == 0 && nc) atomicAdd(cells, nc);
-  }
-  if (order.pairs) {
-    unsigned pb = pair_bits(bits);
-    const int c = __popc(pb);
-    int v = c;  // inclusive scan over the wave's lanes
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int y = __shfl_up(v, d);
-      if (lane >= d) v += y;
-    }
-    if (lane == 63) wsum[wave] = v;
-    __syncthreads();
-    int pos = boff[blockIdx.x] + v - c;
-    for (int w = 0; w < wave; ++w) pos += wsum[w];
-    while (pb) {
-      const int j = __builtin_ctz(pb);  // even: the pair's first cell
-      pb &= pb - 1;
-      list[pos++] = int(uint32_t((first + j) >> 1) | (((bits >> j) & 3u) << 30));
-    }
-    return;
-  }
-  const int c = __popc(bits);
+  unsigned sel = order.pairs ? pair_bits(bits) : bits;
+  const int c = __popc(sel);
   int v = c;  // inclusive scan over the wave's lanes
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     const int y = __shfl_up(v, d);
     if (lane >= d) v += y;
   }
-  if (lane == 63) wsum[wave] = v;
-  __syncthreads();
-  int pos = boff[blockIdx.x] + v - c;
-  for (int w = 0; w < wave; ++w) pos += wsum[w];
-  if (bits && list_chunk_contiguous(order)) {
+  const int total = __shfl(v, 63);
+  int *row = stage[wave];
+  int k = v - c;
+  if (order.pairs) {
+    while (sel) {
+      const int j = __builtin_ctz(sel);  // even: the pair's first cell
+      sel &= sel - 1;
+      row[k++] = int(uint32_t((first + j) >> 1) | (((bits >> j) & 3u) << 30));
+    }
+  } else if (sel && list_chunk_contiguous(order)) {
     const long base = list_order_cell(order, first);
-    while (bits) {
-      const int j = __builtin_ctz(bits);
-      bits &= bits - 1;
-      list[pos++] = int(base + j);
+    while (sel) {
+      const int j = __builtin_ctz(sel);
+      sel &= sel - 1;
+      row[k++] = int(base + j);
     }
   } else {
-    while (bits) {
-      const int j = __builtin_ctz(bits);
-      bits &= bits - 1;
-      list[pos++] = int(list_order_cell(order, first + j));
+    while (sel) {
+      const int j = __builtin_ctz(sel);
+      sel &= sel - 1;
+      row[k++] = int(list_order_cell(order, first + j));
     }
   }
+  if (lane == 0) wsum[wave] = total;
+  __syncthreads();  // the LDS rows and the wave totals
+  int pos = boff[blockIdx.x];
+  for (int w = 0; w < wave; ++w) pos += wsum[w];
+  for (int q = lane; q < total; q += 64) list[pos + q] = row[q];
 }
 
 // explicit instantiations used by the host dispatch (pf_api.hip)

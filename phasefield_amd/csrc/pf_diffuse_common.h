@@ -2,6 +2,8 @@
 // quasi-equilibrium partition of one cell, face weights, the concentration update of both forms,
 // the full-formula update of one cell and the 16-byte run types.
 #pragma once
+#include <type_traits>
+
 #include "pf_kernels.h"
 
 namespace pfamd {
@@ -27,9 +29,12 @@ __device__ inline void partition_cell(const RawCell<F, P> &r, const DiffConsts<t
   for (int i = 0; i < P; ++i) {
     A v = A(0);
     if (o.ph[i] > A(0)) {
-      A t1 = A(0), t2 = A(0);
+      // the sums start from their first term: 0 + x differs from x only for x = -0, and a sum
+      // that differs by the sign of a zero gives the same v below (t1 holds the term
+      // ph[i] * k(i,i) > 0; c - (+0) == c - (-0) for every c other than -0)
+      A t1 = o.ph[0] * dc.pk[i][0], t2 = o.ph[0] * dc.pc[i][0];
 #pragma unroll
-      for (int j = 0; j < P; ++j) {
+      for (int j = 1; j < P; ++j) {
         t1 += o.ph[j] * dc.pk[i][j];
         t2 += o.ph[j] * dc.pc[i][j];
       }
@@ -44,6 +49,41 @@ template <class A> __device__ inline A face_weight(A prod) {
   A w = prod;
   if (prod != A(0) && prod != A(1)) w = sqrt(prod);
   return w;
+}
+// The correctly rounded FP32 square root for x = +0 and x >= 2^-96: the hardware square root
+// (within 1 ulp) and its two neighbours, the neighbour below taken when x <= s_dn * s, the one
+// above when x > s_up * s (residuals by fma).  This is the compiler's sequence for sqrtf without
+// the two parts that change nothing on that range: the rescaling of inputs below 2^-96 and the
+// select that returns +0 and +inf unchanged (for +0 the residuals keep s = +0: s_dn is a NaN and
+// the +1 ulp residual is +0).  Bitwise equal to sqrtf there.
+__device__ inline float sqrt_rn_unscaled(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = s;
+  if (fmaf(-dn, s, x) <= 0.f) r = dn;
+  if (fmaf(-up, s, x) > 0.f) r = up;
+  return r;
+}
+// The weights of N faces, face_weight of each product.  FP32: the unscaled sequence above, which
+// gives face_weight's bits for products that are 0, 1 or at least 2^-96 (sqrt(0) = 0 and
+// sqrt(1) = 1, so the shortcut is not needed), unless a product of the wavefront lies in
+// (0, 2^-96): then the whole wavefront takes face_weight.  One branch for the N faces, so the N
+// square-root sequences interleave.
+template <class A, int N> __device__ inline void face_weights(const A (&prod)[N], A (&w)[N]) {
+  if constexpr (std::is_same<A, float>::value) {
+    // bits - 1 is below bits(2^-96) - 1 exactly for the products in (0, 2^-96)
+    uint32_t m = 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < N; ++i) m = min(m, __float_as_uint(prod[i]) - 1u);
+    if (__builtin_expect(__ballot(m < 0x0f800000u - 1u) == 0, 1)) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) w[i] = sqrt_rn_unscaled(prod[i]);
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) w[i] = face_weight<A>(prod[i]);
 }
 
 // Store q of cell i while other threads may store the other cells of its 32-bit word: each thread
@@ -135,11 +175,17 @@ template <class F, int P> struct DelAcc {
     if constexpr (F::kMixed) s[p] += term;
     else s[0] += term * dc.D[p];
   }
+  // the same for the first face: the mixed form's per-phase sums start from it (0 + x is x but
+  // for x = -0, and a del that differs only by the sign of a zero gives the same value and code)
+  __device__ void first(int p, A term, const DiffConsts<A> &dc) {
+    if constexpr (F::kMixed) s[p] = term;
+    else s[0] += term * dc.D[p];
+  }
   __device__ A result(const DiffConsts<A> &dc) const {
     if constexpr (F::kMixed) {
-      A del = A(0);
+      A del = s[0] * (dc.D[0] * dc.r);
 #pragma unroll
-      for (int p = 0; p < P; ++p) del += s[p] * (dc.D[p] * dc.r);
+      for (int p = 1; p < P; ++p) del += s[p] * (dc.D[p] * dc.r);
       return del;
     } else {
       return s[0] * dc.r;
@@ -168,15 +214,24 @@ __device__ inline typename F::A stencil_del(const StencilLoad<F, P, NF> &s,
   using A = typename F::A;
   CellPF<A> me;
   partition_cell<F, P>(s.me, dc, me);
-  DelAcc<F, P> acc;
+  CellPF<A> nb[NF];
+  A prod[NF * P], w[NF * P];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    CellPF<A> nb;
-    partition_cell<F, P>(s.nb[f], dc, nb);
+    partition_cell<F, P>(s.nb[f], dc, nb[f]);
 #pragma unroll
-    for (int p = 0; p < P; ++p)
-      acc.add(p, face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]), dc);
+    for (int p = 0; p < P; ++p) prod[f * P + p] = nb[f].ph[p] * me.ph[p];
   }
+  face_weights<A>(prod, w);
+  DelAcc<F, P> acc;
+#pragma unroll
+  for (int f = 0; f < NF; ++f)
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const A term = w[f * P + p] * (nb[f].c[p] - me.c[p]);
+      if (f == 0) acc.first(p, term, dc);
+      else acc.add(p, term, dc);
+    }
   return acc.result(dc);
 }
 // ... and the update

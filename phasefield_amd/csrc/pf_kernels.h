@@ -26,8 +26,8 @@ __global__ void sweep_classify_kernel(const uint16_t *__restrict__ id1,
                                       const typename F::T *__restrict__ phi1,
                                       uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
                                       uint8_t *__restrict__ sflag, const int history,
-                                      const double cutoff, const int nx, const int ny,
-                                      const int nz, const long ncell);
+                                      const double cutoff, const float cutoff_f, const int nx,
+                                      const int ny, const int nz, const long ncell);
 
 template <class F, int P, int NF, bool NOFLUX, int TY>
 __global__ void diffuse_kernel(const typename F::C *__restrict__ cin,
@@ -122,21 +122,38 @@ struct ListOrder {
   bool pairs = false;
 };
 // storage index of list-order position p (0 <= p < nx * ny * nz)
+// (every quantity is below 2^31, the limit on a domain's cells, so the divisions are 32-bit: a
+// 64-bit division is several times longer on the device)
 __host__ __device__ inline long list_order_cell(const ListOrder &o, long p) {
   if (o.rows == 0) return p;
-  const long column = long(o.rows) * o.nx * o.nz;  // cells of one full tile
-  const int ty = int(p / column);
-  const int h = o.ny - ty * o.rows < o.rows ? o.ny - ty * o.rows : o.rows;
-  const long q = p - ty * column;
-  const long plane = long(h) * o.nx;
-  const int z = int(q / plane);
-  const long r = q - z * plane;
-  const int yy = int(r / o.nx);
-  const int x = int(r - long(yy) * o.nx);
-  return (long(z) * o.ny + long(ty) * o.rows + yy) * o.nx + x;
+  const uint32_t up = uint32_t(p), nx = uint32_t(o.nx), rows = uint32_t(o.rows);
+  const uint32_t column = rows * nx * uint32_t(o.nz);  // cells of one full tile
+  const uint32_t ty = up / column;
+  const uint32_t left = uint32_t(o.ny) - ty * rows;  // rows of the last tile may be fewer
+  const uint32_t h = left < rows ? left : rows;
+  const uint32_t q = up - ty * column;
+  const uint32_t plane = h * nx;
+  const uint32_t z = q / plane;
+  const uint32_t r = q - z * plane;
+  const uint32_t yy = r / nx;
+  const uint32_t x = r - yy * nx;
+  return (long(z) * o.ny + long(ty * rows + yy)) * o.nx + x;
 }
-// `cells` (optional): the count kernel sets it to 0 and the scatter kernel adds the listed cells to
-// it, for lists whose entries are not cells (pairs)
+// Whether a neighbour's row of four 16-bit ids (slot 0 in the low bits) breaks a single-entry
+// active-set union of grain G: its ids are read in slot order up to the first 0 and each one read
+// must be G, so slot s breaks it when it is neither 0 nor G and every slot before it is a live G
+// (G != 0).  Branch-free: the wavefront's cells evaluate it together.
+__host__ __device__ inline bool id_row_breaks(uint64_t w, uint32_t G) {
+  const bool live_g = G != 0u;
+  const uint32_t lo = uint32_t(w), hi = uint32_t(w >> 32);
+  const uint32_t i0 = lo & 0xffffu, i1 = lo >> 16, i2 = hi & 0xffffu, i3 = hi >> 16;
+  const bool e0 = live_g & (i0 == G), e1 = live_g & (i1 == G), e2 = live_g & (i2 == G);
+  const bool b0 = (i0 != G) & (i0 != 0u), b1 = (i1 != G) & (i1 != 0u);
+  const bool b2 = (i2 != G) & (i2 != 0u), b3 = (i3 != G) & (i3 != 0u);
+  return b0 | (e0 & b1) | (e0 & e1 & b2) | (e0 & e1 & e2 & b3);
+}
+// `cells` (optional): the count kernel's per-block counts of listed cells, for lists whose
+// entries are not cells (pairs); general_scan_kernel over them gives the total
 __global__ void general_count_kernel(const uint8_t *__restrict__ flag,
                                      const unsigned *__restrict__ count, long ncell,
                                      int *__restrict__ bcount, const ListOrder order,
@@ -146,7 +163,7 @@ __global__ void general_scan_kernel(int *__restrict__ bcount, int nb,
 __global__ void general_scatter_kernel(const uint8_t *__restrict__ flag,
                                        const unsigned *__restrict__ count, long ncell,
                                        const int *__restrict__ boff, int *__restrict__ list,
-                                       const ListOrder order, int *__restrict__ cells);
+                                       const ListOrder order);
 
 // Bulk flags of a value_phase field into `flag` and the bulk cells per phase into count[1..3]
 // (two kernels; `unit` is an N-byte scratch array)

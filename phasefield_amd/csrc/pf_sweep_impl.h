@@ -21,6 +21,8 @@
 // [entry][thread] arrays so the data-dependent match/append/sort indices stay cheap; an entry's
 // own value is read from the thread's slot registers.
 #pragma once
+#include <type_traits>
+
 #include "pf_kernels.h"
 
 namespace pfamd {
@@ -459,12 +461,14 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
 // flag 0 for the listed full sweep.  sflag holds 1 for a cell copied by the previous sweep: its
 // destination row already equals its source row then (the buffers swap every sweep and a copy
 // makes them equal), so the write is skipped when `history` is set.
+// The FP32 form compares its slots with cutoff_f, the largest float not above the cutoff: for a
+// float v, v > cutoff_f exactly when double(v) > cutoff.
 template <class F, int K, int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void sweep_classify_kernel(
     const uint16_t *__restrict__ id1, const typename F::T *__restrict__ phi1,
     uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2, uint8_t *__restrict__ sflag,
-    const int history, const double cutoff, const int nx, const int ny, const int nz,
-    const long ncell) {
+    const int history, const double cutoff, const float cutoff_f, const int nx, const int ny,
+    const int nz, const long ncell) {
   using T = typename F::T;
   // Workgroup b runs on XCD b % 8.  With the grid a multiple of 128 workgroups, XCD j takes the
   // groups of 16 consecutive workgroups (4096 cells: 8 rows at nx = 512) numbered j, j + 8, ...:
@@ -477,34 +481,63 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
   }
   const long cell = blk * 256 + threadIdx.x;
   if (cell >= ncell) return;
+  // Every load is issued before any of them is used: the own ids and slots, the history byte and
+  // (K = 4) the six neighbour id rows, so a thread waits on memory once.
   uint32_t g0[K];
   T v0[K];
   load_ids<K>(id1, cell, g0);
+  load_row<T, K>(phi1, cell, v0);
+  const uint8_t prev = sflag[cell];  // 1: copied by the previous sweep (read even without history)
+  // the six neighbours (wrap(x - 1, nx) + nx * y + plane * z and so on) as 32-bit offsets from the
+  // cell: a domain holds fewer than 2^31 cells
+  int x, y, z;
+  cell_coords(cell, nx, ny, x, y, z);
+  const uint32_t c = uint32_t(cell), unx = uint32_t(nx), plane = unx * uint32_t(ny);
+  const uint32_t vol = plane * uint32_t(nz);
+  uint32_t nb[6];
+  nb[0] = x == 0 ? c + unx - 1u : c - 1u;
+  nb[1] = x == nx - 1 ? c + 1u - unx : c + 1u;
+  nb[2] = y == 0 ? c + plane - unx : c - unx;
+  nb[3] = y == ny - 1 ? c + unx - plane : c + unx;
+  nb[4] = z == 0 ? (NOFLUX ? c : c + vol - plane) : c - plane;
+  nb[5] = z == nz - 1 ? (NOFLUX ? c : c + plane - vol) : c + plane;
+  uint64_t w[NF];
+  if constexpr (K == 4) {
+    const uint64_t *rows = reinterpret_cast<const uint64_t *>(id1);
 #pragma unroll
-  for (int s = 0; s < K; ++s) v0[s] = phi1[cell * K + s];
+    for (int f = 0; f < NF; ++f) w[f] = rows[nb[f]];
+  }
+  // keep the arithmetic on the loaded values below every load (the scheduler would otherwise
+  // move a compare of the own slots above the neighbour loads and wait for it there)
+  __builtin_amdgcn_sched_barrier(0);
   int n0 = 0;
   uint32_t G = 0;
 #pragma unroll
-  for (int s = 0; s < K; ++s)
-    if (double(v0[s]) > cutoff) {
+  for (int s = 0; s < K; ++s) {
+    bool above;
+    if constexpr (std::is_same<T, float>::value) above = v0[s] > cutoff_f;
+    else above = double(v0[s]) > cutoff;
+    if (above) {
       ++n0;
       G = g0[s];
     }
+  }
   bool copy = n0 <= 1;
-  bool any = n0 == 1;
-  if (copy) {
-    int x, y, z;
-    cell_coords(cell, nx, ny, x, y, z);
-    const long plane = long(nx) * ny;
-    long nb[6];
-    nb[0] = wrap(x - 1, nx) + long(nx) * y + plane * z;
-    nb[1] = wrap(x + 1, nx) + long(nx) * y + plane * z;
-    nb[2] = x + long(nx) * wrap(y - 1, ny) + plane * z;
-    nb[3] = x + long(nx) * wrap(y + 1, ny) + plane * z;
-    const int zm = NOFLUX ? (z == 0 ? z : z - 1) : wrap(z - 1, nz);
-    const int zp = NOFLUX ? (z == nz - 1 ? z : z + 1) : wrap(z + 1, nz);
-    nb[4] = x + long(nx) * y + plane * zm;
-    nb[5] = x + long(nx) * y + plane * zp;
+  if constexpr (K == 4) {
+    // A neighbour's ids are read in slot order up to its first 0, and each one read must be G
+    // (with no own entry, G is the first id read: slot 0 of the first neighbour whose slot 0 is not
+    // 0).  Four ids are one 64-bit row (id_row_breaks).  Evaluated for every cell, without a branch.
+    if (n0 == 0) {
+#pragma unroll
+      for (int f = NF - 1; f >= 0; --f)
+        if ((w[f] & 0xffffu) != 0u) G = uint32_t(w[f] & 0xffffu);
+    }
+    bool bad = !(n0 == 1 || G != 0u);  // no id at all: not a single-entry union
+#pragma unroll
+    for (int f = 0; f < NF; ++f) bad = bad | id_row_breaks(w[f], G);
+    copy = copy && !bad;
+  } else if (copy) {
+    bool any = n0 == 1;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       uint32_t gn[K];
@@ -526,7 +559,7 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
     copy = copy && any;
   }
   if (copy) {
-    if (!(history && sflag[cell] != 0)) {
+    if (!(history && prev != 0)) {
       using V = typename IdRowT<K>::V;
       reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
 #pragma unroll
@@ -552,13 +585,13 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
   PF_INST_SWEEP(F, K, P, 6, true, false, true)
 #define PF_INST_SWEEP_CLASSIFY(F, K)                                                        \
   template __global__ void sweep_classify_kernel<F, K, 4, false>(                          \
-      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, int, int, \
-      int, long);                                                                          \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, float, int, \
+      int, int, long);                                                                     \
   template __global__ void sweep_classify_kernel<F, K, 6, false>(                          \
-      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, int, int, \
-      int, long);                                                                          \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, float, int, \
+      int, int, long);                                                                     \
   template __global__ void sweep_classify_kernel<F, K, 6, true>(                           \
-      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, int, int, \
-      int, long);
+      const uint16_t *, const F::T *, uint16_t *, F::T *, uint8_t *, int, double, float, int, \
+      int, int, long);
 
 }  // namespace pfamd

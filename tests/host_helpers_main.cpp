@@ -10,6 +10,8 @@
 //     mixed_conc_bytes leaves room for every code and for the 32-bit words holding them.
 //  3. comp_encode / comp_decode (the device's rounding of the Kahan compensation) against the
 //     oracle's independent quantize_comp (orc_quantize_comp) on random and edge inputs.
+//  4. id_row_breaks (the copy-cell rule on a neighbour's four ids as one 64-bit word) against the
+//     slot-by-slot walk it replaces, on rows drawn from ids that include 0 and G.
 // Exit status 0 when every check holds.
 #include <cmath>
 #include <cstdint>
@@ -128,8 +130,54 @@ static void code_rounding() {
   std::printf("code rounding: %ld cases equal to the oracle\n", checked);
 }
 
+// the copy-cell rule of one neighbour row as a slot walk (the K = 6 path of the classification and
+// the reference's loop): ids up to the first 0 are live; with a grain already known every live id
+// must equal it, otherwise the first live id becomes the grain
+static bool walk_breaks(const uint32_t (&id)[4], uint32_t &G, bool &any) {
+  bool bad = false;
+  for (int s = 0; s < 4 && id[s] != 0; ++s) {
+    if (!any) {
+      G = id[s];
+      any = true;
+    } else if (id[s] != G) {
+      bad = true;
+    }
+  }
+  return bad;
+}
+
+static void id_rows() {
+  std::mt19937 rng(5);
+  long checked = 0;
+  const uint32_t pool[] = {0u, 1u, 2u, 7u, 0xffffu, 0x8000u};
+  for (int k = 0; k < 400000; ++k) {
+    uint32_t id[4];
+    for (int s = 0; s < 4; ++s) id[s] = pool[rng() % 6];
+    const uint64_t w = uint64_t(id[0]) | uint64_t(id[1]) << 16 | uint64_t(id[2]) << 32 |
+                       uint64_t(id[3]) << 48;
+    for (uint32_t G : {0u, 1u, 2u, 7u, 0xffffu}) {
+      // a known grain G (own entry, or an earlier neighbour; G = 0 only as an own slot's id)
+      uint32_t g = G;
+      bool any = true;
+      CHECK(walk_breaks(id, g, any) == id_row_breaks(w, G), "row %016llx G %u",
+            (unsigned long long)w, G);
+      ++checked;
+    }
+    // no grain known yet: the walk takes slot 0 (when not 0) as the grain, as the kernel does
+    uint32_t g = 0;
+    bool any = false;
+    const bool walk = walk_breaks(id, g, any);
+    const uint32_t G = id[0];
+    CHECK(walk == (G != 0u && id_row_breaks(w, G)), "row %016llx first grain", (unsigned long long)w);
+    CHECK(any == (G != 0u), "row %016llx live", (unsigned long long)w);
+    ++checked;
+  }
+  std::printf("id rows: %ld cases equal to the slot walk\n", checked);
+}
+
 int main() {
   list_order();
+  id_rows();
   code_storage();
   code_rounding();
   std::printf("host_helpers_main: %s\n", g_fail ? "FAILED" : "ok");

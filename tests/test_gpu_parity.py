@@ -512,3 +512,66 @@ def test_sweep_history_across_restore(tmp_path):
     a.upload(*st)  # upload after history: starts over
     a.step(4)
     assert_same(a.download(), b.download(), 4, "re-upload")
+
+
+def _host_diffusion_listed(phase, shape, P, noflux, params, mixed):
+    """Cells the diffusion sub-step lists with the plane-marching kernel, counted on the host from
+    the stored value_phase: a cell is bulk of phase p when its value_phase and its six neighbours'
+    are exactly e_p and phase p's partition is the identity for every solute; every cell that is not
+    bulk of the bulk phase (the phase with the most bulk cells, the first on a tie) is listed."""
+    nx, ny, nz = shape
+    pp = oracle.as_orc_params(params)
+    conv = np.float32 if mixed else np.float64
+    mask = [q + 1 for q in range(P)
+            if all(conv(pp.part_k[s][q][q]) == 1 and conv(pp.part_c[s][q][q]) == 0
+                   for s in range(1))]
+    ph = phase.reshape(nz, ny, nx, 3)[..., :P]
+    ones = (ph == 1.0).sum(-1)
+    zeros = (ph == 0.0).sum(-1)
+    unit = np.where((ones == 1) & (zeros == P - 1), np.argmax(ph == 1.0, axis=-1) + 1, 0)
+    unit = np.where(np.isin(unit, mask), unit, 0)
+    flag = unit.copy()
+    for axis in (2, 1, 0):
+        for d in (-1, 1):
+            if axis == 0 and noflux:
+                nb = (np.concatenate([unit[:1], unit[:-1]]) if d < 0
+                      else np.concatenate([unit[1:], unit[-1:]]))
+            else:
+                nb = np.roll(unit, -d, axis)
+            flag = np.where(nb == unit, flag, 0)
+    counts = [int((flag == q).sum()) for q in (1, 2, 3)]
+    bulk = 0
+    for q in (1, 2):
+        if counts[q] > counts[bulk]:
+            bulk = q
+    return nx * ny * nz - counts[bulk]
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("bc", [0, 1])
+def test_diffusion_list_length(tmp_path, form, bc):
+    """The diffusion list's cell count (pf_list_lengths) against a host count from the downloaded
+    value_phase.  The mixed form's list holds byte pairs and counts its cells per list block, summed
+    when asked: asking twice gives the same count."""
+    shape = (64, 20, 12)
+    banner(f"diffusion list length form={form} bc={bc}")
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=5, nuclei=4,
+                            bc_z="noflux" if bc else "periodic")
+    g = Domain(nx, ny, nz, depth=4, nphase=2, nsolute=1,
+               precision="mixed" if form == oracle.ORC_MIXED else "fp64", bc_z=bc)
+    pp = oracle.as_orc_params(p)
+    from phasefield_amd._capi import PfParams
+    gp = PfParams.from_buffer_copy(pp)
+    g.set_params(gp)
+    g.upload(*st)
+    g.step(1)  # a sweep, then sub-steps: the list of the stored value_phase
+    g.diffuse(0, 1, gp.dt_d[0])
+    _, listed = g.list_lengths()
+    _, again = g.list_lengths()
+    gs = g.download()
+    g.close()
+    want = _host_diffusion_listed(gs["phase"], shape, 2, bool(bc), p, form == oracle.ORC_MIXED)
+    assert 0 < want < nx * ny * nz
+    assert listed == want, f"listed {listed}, host count {want}"
+    assert again == listed
