@@ -493,7 +493,9 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
   } else {
     const unsigned nb = blocks_for(h->N);
     constexpr int V = F::kMixed ? 4 : 2;  // 16-byte runs per lane
+    // (the marching kernel addresses a plane with 32-bit byte offsets)
     const bool march = h->diff_split && h->diff_vec && h->diff_march && h->d.nx % V == 0 &&
+                       long(h->d.nx) * h->d.ny * long(sizeof(typename F::C)) < (1L << 32) &&
                        (h->diff_by < 0 || h->diff_by == 8 || h->diff_by == 16 || h->diff_by == 32);
     if (!h->flag_valid || h->flag_nf != NF) {
       const hipError_t ec =

@@ -545,27 +545,33 @@ __device__ inline void vec_update<FormF64, 2>(const double *__restrict__, double
   o.v[1] = c.v[1] + del[1];
   store_nt(reinterpret_cast<VecA<double, 2> *>(out + i), o);
 }
-template <>
-__device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, float *__restrict__ out,
-                                                long i, long n, const VecA<float, 4> &c,
-                                                const float (&del)[4], const VecCode<4> &qi,
-                                                uint32_t key) {
+// The mixed form's update of a run of four cells i..i+3: values stored at vdst, their four 4-bit
+// codes (one 16-bit word) at tdst; hbase = i * kCompCellMul gives the rounding hashes of comp_hash
+// by additions.
+__device__ inline void run_update_mixed(float *__restrict__ vdst, uint16_t *__restrict__ tdst,
+                                        const VecA<float, 4> &c, const float (&del)[4],
+                                        const VecCode<4> &qi, uint32_t key, uint32_t hbase) {
   VecA<float, 4> o;
-  VecCode<4> qo;
-  // rounding hashes of cells i..i+3 (comp_hash) from one product
-  const uint32_t base = uint32_t(i) * kCompCellMul;
-  const uint32_t h[4] = {key ^ base, key ^ (base + kCompCellMul), key ^ (base + 2u * kCompCellMul),
-                         key ^ (base + 3u * kCompCellMul)};
+  uint32_t w = 0;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const float comp = comp_decode(qi.q[v], c.v[v]);
     const float y = del[v] - comp;
     const float t = c.v[v] + y;
     o.v[v] = t;
-    qo.q[v] = comp_encode((t - c.v[v]) - y, t, h[v]);
+    const CompCode q = comp_encode((t - c.v[v]) - y, t, key ^ (hbase + uint32_t(v) * kCompCellMul));
+    w |= (uint32_t(q) & 15u) << (4 * v);
   }
-  store_nt(reinterpret_cast<VecA<float, 4> *>(out + i), o);
-  store_codes4_nt(out, n, i, qo);
+  store_nt(reinterpret_cast<VecA<float, 4> *>(vdst), o);
+  __builtin_nontemporal_store(uint16_t(w), tdst);
+}
+template <>
+__device__ inline void vec_update<FormMixed, 4>(const float *__restrict__ in, float *__restrict__ out,
+                                                long i, long n, const VecA<float, 4> &c,
+                                                const float (&del)[4], const VecCode<4> &qi,
+                                                uint32_t key) {
+  run_update_mixed(out + i, reinterpret_cast<uint16_t *>(comp_bytes(out, n) + (i >> 1)), c, del,
+                   qi, key, uint32_t(i) * kCompCellMul);
 }
 
 template <class F, int P, int NF, bool NOFLUX, int BY, int V>
@@ -754,16 +760,27 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
     return z < 0 ? z + nz : (z >= nz ? z - nz : z);
   };
   auto ldv = [&](int z, int o) -> VA { return *reinterpret_cast<const VA *>(cin + plane * z + o); };
-  auto ldrows = [&](int z, Rows &d) {
+  // A lane's own run in plane z: the plane's base is wave-uniform and the run's offset a 32-bit
+  // byte count (the launcher keeps a plane below 2^32 bytes), so the address is a scalar base plus
+  // a vector offset with no 64-bit arithmetic per plane.  Mixed: plane is even (nx % 4 == 0), so the
+  // code byte of cell plane * z + o is byte plane * z / 2 + o / 2 of the code region.
+  uint32_t ob[R], ot[R], oh[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) d.c[r] = ldv(z, oc[r]);
+  for (int r = 0; r < R; ++r) {
+    ob[r] = uint32_t(oc[r]) * uint32_t(sizeof(typename F::C));
+    ot[r] = uint32_t(oc[r]) >> 1;
+    oh[r] = uint32_t(oc[r]) * kCompCellMul;
+  }
+  auto ldrows = [&](int z, Rows &d) {
+    const char *pb = reinterpret_cast<const char *>(cin + plane * z);
+#pragma unroll
+    for (int r = 0; r < R; ++r) d.c[r] = *reinterpret_cast<const VA *>(pb + ob[r]);
   };
   auto ldtags = [&](int z, Tags &d) {
     if constexpr (F::kMixed) {
+      const uint8_t *tb = comp_bytes(cin, ncell) + (plane * z >> 1);
 #pragma unroll
-      for (int r = 0; r < R; ++r)
-        d.t[r].w = *reinterpret_cast<const uint16_t *>(comp_bytes(cin, ncell) +
-                                                       ((plane * z + oc[r]) >> 1));
+      for (int r = 0; r < R; ++r) d.t[r].w = *reinterpret_cast<const uint16_t *>(tb + ot[r]);
     }
   };
   // Register rings, indexed by the plane's position in the chunk modulo 4: planes z-1, z, z+1 are
@@ -781,6 +798,16 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
   lds_barrier();
   // one plane: pm, pc, pp = planes z-1, z, z+1; pn receives plane z+2; tc = tags of plane z, tn
   // receives the tags of plane z+2
+  // dz carries c(z) - c(z-1) of the lane's cells into the plane step and leaves c(z+1) - c(z) for
+  // the next: the -z term of plane z + 1 is the negated +z term of plane z (a - b == -(b - a)
+  // exactly), one subtraction per cell fewer.  Likewise along x inside a run.
+  Rows dz;
+  if (NF == 6) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int v = 0; v < V; ++v) dz.c[r].v[v] = r1.c[r].v[v] - r0.c[r].v[v];
+  }
   auto step = [&](const int z, const Rows &pm, const Rows &pc, const Rows &pp, Rows &pn,
                   const Tags &tc, Tags &tn) {
     const VA *sp = &s[(z - z0) & 1][0][0];
@@ -824,27 +851,46 @@ __global__ __launch_bounds__(64 * BY / R) void diffuse_bulk_march_kernel(
 #pragma unroll
         for (int v = 0; v < V; ++v) qc.q[v] = comp_unpack(tg.w >> (4 * v));
       }
+      // +x differences of the run (the last one against the next lane's first cell) and the +z
+      // differences, for every lane (the next plane needs them whether or not this row stores)
+      A dx[V], dzp[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const A c = pc.c[r].v[v];
+        dx[v] = (v == V - 1 ? xp[r] : pc.c[r].v[v + 1 < V ? v + 1 : v]) - c;
+        if (NF == 6) dzp[v] = pp.c[r].v[v] - c;
+      }
       if (act[r]) {
         A del[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           const A c = pc.c[r].v[v];
-          const A l = v == 0 ? xm[r] : pc.c[r].v[v > 0 ? v - 1 : 0];
-          const A rr = v == V - 1 ? xp[r] : pc.c[r].v[v + 1 < V ? v + 1 : v];
           BulkAcc<F> d;
-          d.add(l - c, Dp);
-          d.add(rr - c, Dp);
+          d.first(v == 0 ? xm[r] - c : -dx[v > 0 ? v - 1 : 0], Dp);  // l - c
+          d.add(dx[v], Dp);                                        // r - c
           if (NF >= 4) {
             d.add(ym[r].v[v] - c, Dp);
             d.add(yp[r].v[v] - c, Dp);
           }
           if (NF == 6) {
-            d.add(pm.c[r].v[v] - c, Dp);
-            d.add(pp.c[r].v[v] - c, Dp);
+            d.add(-dz.c[r].v[v], Dp);  // c(z-1) - c
+            d.add(dzp[v], Dp);         // c(z+1) - c
           }
           del[v] = d.result(dc.r, Dpr);
         }
-        vec_update<F, V>(cin, cout, plane * z + oc[r], ncell, pc.c[r], del, qc, dc.key);
+        char *vb = reinterpret_cast<char *>(cout + plane * z) + ob[r];
+        if constexpr (F::kMixed) {
+          uint8_t *tb = comp_bytes(cout, ncell) + (plane * z >> 1) + ot[r];
+          run_update_mixed(reinterpret_cast<float *>(vb), reinterpret_cast<uint16_t *>(tb),
+                           pc.c[r], del, qc, dc.key, uint32_t(plane * z) * kCompCellMul + oh[r]);
+        } else {
+          vec_update<F, V>(cin, reinterpret_cast<typename F::C *>(vb), 0, ncell, pc.c[r], del, qc,
+                           dc.key);
+        }
+      }
+      if (NF == 6) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) dz.c[r].v[v] = dzp[v];
       }
     }
     // plane z+1 into the other buffer (unread after the chunk's last plane); its readers wait at

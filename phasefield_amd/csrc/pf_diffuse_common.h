@@ -201,6 +201,12 @@ template <class F> struct BulkAcc {
     if constexpr (F::kMixed) s += diff;
     else s += diff * Dp;
   }
+  // the first face's term: the sum starts from it (0 + x is x but for x = -0, and a sum that
+  // differs only by the sign of a zero gives the same update)
+  __device__ void first(A diff, A Dp) {
+    if constexpr (F::kMixed) s = diff;
+    else s = diff * Dp;
+  }
   __device__ A result(A r, A Dpr) const {
     if constexpr (F::kMixed) return s * Dpr;
     else return s * r;
