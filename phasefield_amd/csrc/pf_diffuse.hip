@@ -305,13 +305,6 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict
   }
 }
 
-// 0x80 in each byte of x that is 0, 0 in the others: the low seven bits of a byte plus 0x7f set its
-// top bit unless they are all 0, and no sum carries into the next byte
-__device__ inline uint32_t zero_bytes(uint32_t x) {
-  const uint32_t t = (x & 0x7f7f7f7fu) + 0x7f7f7f7fu;
-  return ~(t | x | 0x7f7f7f7fu);
-}
-
 // The same for nx % 4 == 0, four x-consecutive cells per thread: their own bytes, their -y/+y and
 // -z/+z neighbours are one aligned 32-bit word each, and only the two x ends are single bytes.
 template <int NF, bool NOFLUX>

@@ -139,6 +139,12 @@ __host__ __device__ inline long list_order_cell(const ListOrder &o, long p) {
   const uint32_t x = r - yy * nx;
   return (long(z) * o.ny + long(ty * rows + yy)) * o.nx + x;
 }
+// 0x80 in each byte of x that is 0, 0 in the others: the low seven bits of a byte plus 0x7f set its
+// top bit unless they are all 0, and no sum carries into the next byte
+__host__ __device__ inline uint32_t zero_bytes(uint32_t x) {
+  const uint32_t t = (x & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+  return ~(t | x | 0x7f7f7f7fu);
+}
 // Whether a neighbour's row of four 16-bit ids (slot 0 in the low bits) breaks a single-entry
 // active-set union of grain G: its ids are read in slot order up to the first 0 and each one read
 // must be G, so slot s breaks it when it is neither 0 nor G and every slot before it is a live G

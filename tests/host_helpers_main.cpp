@@ -12,6 +12,7 @@
 //     oracle's independent quantize_comp (orc_quantize_comp) on random and edge inputs.
 //  4. id_row_breaks (the copy-cell rule on a neighbour's four ids as one 64-bit word) against the
 //     slot-by-slot walk it replaces, on rows drawn from ids that include 0 and G.
+//  5. zero_bytes (the diffusion flag pass's carry-free zero-byte test) against a byte loop.
 // Exit status 0 when every check holds.
 #include <cmath>
 #include <cstdint>
@@ -175,9 +176,33 @@ static void id_rows() {
   std::printf("id rows: %ld cases equal to the slot walk\n", checked);
 }
 
+// zero_bytes (the flag pass's byte-wise equality test) against a byte loop
+static void zero_byte_words() {
+  std::mt19937 rng(9);
+  long checked = 0;
+  auto one = [&](uint32_t x) {
+    uint32_t want = 0;
+    for (int b = 0; b < 4; ++b)
+      if (((x >> (8 * b)) & 0xffu) == 0u) want |= 0x80u << (8 * b);
+    CHECK(zero_bytes(x) == want, "zero_bytes(%08x) = %08x, want %08x", x, zero_bytes(x), want);
+    ++checked;
+  };
+  // every word whose bytes are drawn from {0, 1, 0x7f, 0x80, 0xff, random}
+  const uint32_t pick[] = {0u, 1u, 0x7fu, 0x80u, 0xffu};
+  for (int k = 0; k < 6 * 6 * 6 * 6; ++k) {
+    uint32_t x = 0;
+    int r = k;
+    for (int b = 0; b < 4; ++b, r /= 6) x |= (r % 6 < 5 ? pick[r % 6] : (rng() & 0xffu)) << (8 * b);
+    one(x);
+  }
+  for (int k = 0; k < 200000; ++k) one(uint32_t(rng()) & uint32_t(rng() | rng()));
+  std::printf("zero bytes: %ld words equal to the byte loop\n", checked);
+}
+
 int main() {
   list_order();
   id_rows();
+  zero_byte_words();
   code_storage();
   code_rounding();
   std::printf("host_helpers_main: %s\n", g_fail ? "FAILED" : "ok");
