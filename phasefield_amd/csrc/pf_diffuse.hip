@@ -262,6 +262,26 @@ __global__ __launch_bounds__(256) void unit_phase4_kernel(const T *__restrict__ 
 
 constexpr int kClassifyBlocks = 4096;  // grid-stride, so the per-phase counts take few atomics
 
+// A workgroup's per-phase bulk counts (wave-uniform n1..n3 in each of its four wavefronts) added to
+// count[1..3] with one atomic per phase: atomics on one address serialise, and one per wavefront
+// (3 x 16384 at 512^3) cost as much as the pass itself.  Every thread of the workgroup calls it.
+__device__ inline void add_phase_counts(unsigned n1, unsigned n2, unsigned n3,
+                                        unsigned *__restrict__ count) {
+  __shared__ unsigned part[4][3];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[wave][0] = n1;
+    part[wave][1] = n2;
+    part[wave][2] = n3;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const unsigned n = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                       part[3][threadIdx.x];
+    if (n) atomicAdd(count + 1 + threadIdx.x, n);
+  }
+}
+
 template <int NF, bool NOFLUX>
 __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict__ unit,
                                                        uint8_t *__restrict__ flag, const int nx,
@@ -298,11 +318,7 @@ __global__ __launch_bounds__(256) void classify_kernel(const uint8_t *__restrict
     n2 += unsigned(__popcll(__ballot(p == 2)));
     n3 += unsigned(__popcll(__ballot(p == 3)));
   }
-  if ((threadIdx.x & 63) == 0) {
-    if (n1) atomicAdd(count + 1, n1);
-    if (n2) atomicAdd(count + 2, n2);
-    if (n3) atomicAdd(count + 3, n3);
-  }
+  add_phase_counts(n1, n2, n3, count);
 }
 
 // The same for nx % 4 == 0, four x-consecutive cells per thread: their own bytes, their -y/+y and
@@ -325,27 +341,27 @@ __global__ __launch_bounds__(256) void classify4_kernel(const uint8_t *__restric
       const uint32_t q32 = uint32_t(q), c = 4 * q32;
       int x, y, z;
       cell_coords(c, nx, ny, x, y, z);
+      // every load issued together, without a branch on the quad's own bytes (a quad of zeros
+      // gives zeros below)
       const uint32_t me = u32[q32];
-      if (me != 0u) {
-        uint32_t ym = me, yp = me, zm = me, zp = me;
-        if (NF >= 4) {
-          ym = u32[y == 0 ? q32 + qplane - qx : q32 - qx];
-          yp = u32[y == ny - 1 ? q32 + qx - qplane : q32 + qx];
-        }
-        if (NF == 6) {
-          zm = u32[z == 0 ? (NOFLUX ? q32 : q32 + qvol - qplane) : q32 - qplane];
-          zp = u32[z == nz - 1 ? (NOFLUX ? q32 : q32 + qplane - qvol) : q32 + qplane];
-        }
-        const uint32_t xl = unit[x == 0 ? c + uint32_t(nx) - 1u : c - 1u];
-        const uint32_t xr = unit[x + 4 == nx ? c - uint32_t(x) : c + 4u];
-        // byte v of `left` / `right` is the -x / +x neighbour of cell v; a cell keeps its byte when
-        // all its neighbours' bytes equal it (byte-wise, no carry between bytes)
-        const uint32_t left = (me << 8) | xl, right = (me >> 8) | (xr << 24);
-        const uint32_t diff = (me ^ left) | (me ^ right) | (me ^ ym) | (me ^ yp) | (me ^ zm) |
-                              (me ^ zp);
-        const uint32_t same = zero_bytes(diff);  // 0x80 in the bytes of the cells that keep theirs
-        out = me & (same | (same - (same >> 7)));
+      uint32_t ym = me, yp = me, zm = me, zp = me;
+      if (NF >= 4) {
+        ym = u32[y == 0 ? q32 + qplane - qx : q32 - qx];
+        yp = u32[y == ny - 1 ? q32 + qx - qplane : q32 + qx];
       }
+      if (NF == 6) {
+        zm = u32[z == 0 ? (NOFLUX ? q32 : q32 + qvol - qplane) : q32 - qplane];
+        zp = u32[z == nz - 1 ? (NOFLUX ? q32 : q32 + qplane - qvol) : q32 + qplane];
+      }
+      const uint32_t xl = unit[x == 0 ? c + uint32_t(nx) - 1u : c - 1u];
+      const uint32_t xr = unit[x + 4 == nx ? c - uint32_t(x) : c + 4u];
+      // byte v of `left` / `right` is the -x / +x neighbour of cell v; a cell keeps its byte when
+      // all its neighbours' bytes equal it (byte-wise, no carry between bytes)
+      const uint32_t left = (me << 8) | xl, right = (me >> 8) | (xr << 24);
+      const uint32_t diff = (me ^ left) | (me ^ right) | (me ^ ym) | (me ^ yp) | (me ^ zm) |
+                            (me ^ zp);
+      const uint32_t same = zero_bytes(diff);  // 0x80 in the bytes of the cells that keep theirs
+      out = me & (same | (same - (same >> 7)));
       reinterpret_cast<uint32_t *>(flag)[q32] = out;
     }
     int p[4];
@@ -358,11 +374,7 @@ __global__ __launch_bounds__(256) void classify4_kernel(const uint8_t *__restric
       n3 += unsigned(__popcll(__ballot(p[v] == 3)));
     }
   }
-  if ((threadIdx.x & 63) == 0) {
-    if (n1) atomicAdd(count + 1, n1);
-    if (n2) atomicAdd(count + 2, n2);
-    if (n3) atomicAdd(count + 3, n3);
-  }
+  add_phase_counts(n1, n2, n3, count);
 }
 
 template <class T, int P, int NF, bool NOFLUX>
