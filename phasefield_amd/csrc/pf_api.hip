@@ -319,13 +319,13 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true>), dim3(ng), dim3(kSweepBS),
                        0, h->stream, h->id[a], phi1, h->id[b], phi2, static_cast<T *>(h->pfrac),
                        c0, cs, h->d.nx, h->d.ny, h->d.nz, h->N, h->glist, h->gblock + nl,
-                       h->sweep_queue ? h->squeue : static_cast<int *>(nullptr));
+                       h->sweep_queue ? h->squeue : static_cast<int *>(nullptr), h->sflag);
     h->sweep_hist = true;
   } else {
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, ISO, false>),
                        dim3(blocks_for(h->N, kSweepBS)), dim3(kSweepBS), 0, h->stream, h->id[a],
                        phi1, h->id[b], phi2, static_cast<T *>(h->pfrac), c0, cs, h->d.nx, h->d.ny,
-                       h->d.nz, h->N, nullptr, nullptr, nullptr);
+                       h->d.nz, h->N, nullptr, nullptr, nullptr, nullptr);
     h->sweep_hist = false;
   }
   h->cur = b;
@@ -887,6 +887,8 @@ int pf_set_params(pf_handle *h, const pf_params *p) {
     return fail(h, PF_ERR_INVALID, "params.dims does not match the domain shape");
   for (int s = 0; s < h->d.nsolute; ++s)
     if (p->n_sub[s] < 0) return fail(h, PF_ERR_INVALID, "negative sub-step count");
+  // the split sweep's per-cell summaries count own slots above the cutoff
+  if (h->have_params && p->cutoff != h->p.cutoff) h->sweep_hist = false;
   h->p = *p;
   h->have_params = true;
   update_bulk_mask(h);

@@ -16,7 +16,8 @@ __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const typename F::C *__restrict__ conc0,
                                 const PFConsts<typename F::A> cs, const int nx, const int ny,
                                 const int nz, const long ncell, const int *__restrict__ list,
-                                const int *__restrict__ count, int *__restrict__ queue);
+                                const int *__restrict__ count, int *__restrict__ queue,
+                                uint8_t *__restrict__ sflag);
 // the LIST walk's per-XCD counters (queue[32 * xcd], xcd < 8) set to zero: one workgroup of 8
 __global__ void zero_queue_kernel(int *__restrict__ queue);
 
@@ -138,6 +139,14 @@ __host__ __device__ inline long list_order_cell(const ListOrder &o, long p) {
   const uint32_t yy = r / nx;
   const uint32_t x = r - yy * nx;
   return (long(z) * o.ny + long(ty * rows + yy)) * o.nx + x;
+}
+// The split sweep's per-cell byte (sflag).  The classification writes 0 for a cell the listed sweep
+// must update, nonzero for a copy cell; the listed sweep then rewrites its cells' bytes.  Besides
+// bit 0 (copied by this sweep) the byte summarises the cell's new own slots for the next sweep's
+// classification, which then reads no phi: bit 1 when at most one slot is above the cutoff, bit 2
+// when exactly one is, bits 3-5 that slot's index.
+__host__ __device__ inline uint8_t slot_summary(int above, int slot) {
+  return above > 1 ? uint8_t(0) : uint8_t(2u | (above == 1 ? 4u | (unsigned(slot) << 3) : 0u));
 }
 // 0x80 in each byte of x that is 0, 0 in the others: the low seven bits of a byte plus 0x7f set its
 // top bit unless they are all 0, and no sum carries into the next byte

@@ -47,13 +47,49 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     typename F::T *__restrict__ pfrac, const typename F::C *__restrict__ conc0,
     const PFConsts<typename F::A> cs, const int nx, const int ny, const int nz,
     const long ncell, const int *__restrict__ list, const int *__restrict__ count,
-    int *__restrict__ queue) {
+    int *__restrict__ queue, uint8_t *__restrict__ sflag) {
   using T = typename F::T;
   using A = typename F::A;
   __shared__ uint16_t sLg[kStencilMax][kSweepBS];
   __shared__ T sV2[kStencilMax][kSweepBS];
   __shared__ double sDel[kStencilMax][kSweepBS];
   const int t = threadIdx.x;
+  // the cell's new row: ids, slots renormalised by *= 1./sum, value_phase; the listed sweep also
+  // records its own-slot summary for the next classification (slot_summary), from the stored
+  // values compared as the classification compares them
+  auto store_cell = [&](const long cell, const uint32_t (&go)[K], const double (&out)[K],
+                        const double (&vp)[4], const double sum) {
+    const double inv = 1. / sum;
+    store_ids<K>(id2, cell, go);
+    int above = 0, slot = 0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const T v = T(out[s] * inv);
+      phi2[cell * K + s] = v;
+      if (double(v) > cs.cutoff) {
+        ++above;
+        slot = s;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
+    if (LIST) sflag[cell] = slot_summary(above, slot);
+  };
+  // a cell whose union has one entry keeps its row (solver.cpp:207-216)
+  auto copy_cell = [&](const long cell, const T (&v0)[K]) {
+    using V = typename IdRowT<K>::V;
+    reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
+    int above = 0, slot = 0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      phi2[cell * K + s] = v0[s];
+      if (double(v0[s]) > cs.cutoff) {
+        ++above;
+        slot = s;
+      }
+    }
+    if (LIST) sflag[cell] = slot_summary(above, slot);
+  };
   // no block-wide barrier below: threads run their cells independently
   auto body = [&](const long cell) {
   int x, y, z;
@@ -218,10 +254,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     }
     if (!over) {
       if (m == 1) {
-        using V = typename IdRowT<K>::V;
-        reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
-#pragma unroll
-        for (int s = 0; s < K; ++s) phi2[cell * K + s] = v0[s];
+        copy_cell(cell, v0);
         return;
       }
       A dG[4][4];
@@ -308,12 +341,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
           out[s] = 0.0;
         }
       }
-      const double inv = 1. / sum;
-      store_ids<K>(id2, cell, go);
-#pragma unroll
-      for (int s = 0; s < K; ++s) phi2[cell * K + s] = T(out[s] * inv);
-#pragma unroll
-      for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
+      store_cell(cell, go, out, vp, sum);
       return;
     }
   }
@@ -357,10 +385,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   }
 
   if (n == 1) {
-    using V = typename IdRowT<K>::V;
-    reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
-#pragma unroll
-    for (int s = 0; s < K; ++s) phi2[cell * K + s] = v0[s];
+    copy_cell(cell, v0);
     return;
   }
 
@@ -414,12 +439,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
       out[s] = 0.0;
     }
   }
-  const double inv = 1. / sum;
-  store_ids<K>(id2, cell, go);
-#pragma unroll
-  for (int s = 0; s < K; ++s) phi2[cell * K + s] = T(out[s] * inv);
-#pragma unroll
-  for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
+  store_cell(cell, go, out, vp, sum);
   };
   if (LIST) {
     // Workgroup b runs on XCD b % 8: each XCD walks a contiguous eighth of the list (row-tile
@@ -458,9 +478,11 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
 // keeps its slots, value_phase untouched).  The union has one entry exactly when at most one own
 // slot is above the cutoff and every neighbour id before its list's first zero is that grain (or,
 // with no own entry, all those ids are one grain).  Copy cells are written here; the others get
-// flag 0 for the listed full sweep.  sflag holds 1 for a cell copied by the previous sweep: its
-// destination row already equals its source row then (the buffers swap every sweep and a copy
-// makes them equal), so the write is skipped when `history` is set.
+// flag 0 for the listed full sweep.  With `history` (the previous sweep was split like this one
+// and nothing has changed the state since) sflag holds what that sweep left: bit 0 for a cell it
+// copied, whose destination row then already equals its source row (the buffers swap every sweep
+// and a copy makes them equal), so the write is skipped; bits 1-5 the own-slot summary
+// (slot_summary), so the slots are not read at all.
 // The FP32 form compares its slots with cutoff_f, the largest float not above the cutoff: for a
 // float v, v > cutoff_f exactly when double(v) > cutoff.
 template <class F, int K, int NF, bool NOFLUX>
@@ -483,11 +505,13 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
   if (cell >= ncell) return;
   // Every load is issued before any of them is used: the own ids and slots, the history byte and
   // (K = 4) the six neighbour id rows, so a thread waits on memory once.
+  // With history the byte the previous sweep left (slot_summary) says how many own slots are above
+  // the cutoff and which one, so the slots themselves are read only to write a copy.
   uint32_t g0[K];
   T v0[K];
   load_ids<K>(id1, cell, g0);
-  load_row<T, K>(phi1, cell, v0);
-  const uint8_t prev = sflag[cell];  // 1: copied by the previous sweep (read even without history)
+  if (!history) load_row<T, K>(phi1, cell, v0);
+  const uint8_t prev = sflag[cell];
   // the six neighbours (wrap(x - 1, nx) + nx * y + plane * z and so on) as 32-bit offsets from the
   // cell: a domain holds fewer than 2^31 cells
   int x, y, z;
@@ -510,16 +534,25 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
   // keep the arithmetic on the loaded values below every load (the scheduler would otherwise
   // move a compare of the own slots above the neighbour loads and wait for it there)
   __builtin_amdgcn_sched_barrier(0);
-  int n0 = 0;
+  int n0 = 0, slot = 0;
   uint32_t G = 0;
+  if (history) {
+    n0 = (prev & 2) ? ((prev & 4) ? 1 : 0) : 2;  // 2: two or more
+    slot = (prev >> 3) & 7;
 #pragma unroll
-  for (int s = 0; s < K; ++s) {
-    bool above;
-    if constexpr (std::is_same<T, float>::value) above = v0[s] > cutoff_f;
-    else above = double(v0[s]) > cutoff;
-    if (above) {
-      ++n0;
-      G = g0[s];
+    for (int s = 0; s < K; ++s)
+      if (n0 == 1 && s == slot) G = g0[s];
+  } else {
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      bool above;
+      if constexpr (std::is_same<T, float>::value) above = v0[s] > cutoff_f;
+      else above = double(v0[s]) > cutoff;
+      if (above) {
+        ++n0;
+        G = g0[s];
+        slot = s;
+      }
     }
   }
   bool copy = n0 <= 1;
@@ -559,13 +592,14 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
     copy = copy && any;
   }
   if (copy) {
-    if (!(history && prev != 0)) {
+    if (!(history && (prev & 1))) {  // not copied by the previous sweep: the rows differ
       using V = typename IdRowT<K>::V;
       reinterpret_cast<V *>(id2)[cell] = reinterpret_cast<const V *>(id1)[cell];
+      if (history) load_row<T, K>(phi1, cell, v0);
 #pragma unroll
       for (int s = 0; s < K; ++s) phi2[cell * K + s] = v0[s];
     }
-    sflag[cell] = 1;
+    sflag[cell] = uint8_t(1u | slot_summary(n0, slot));
   } else {
     sflag[cell] = 0;
   }
@@ -574,7 +608,7 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
 #define PF_INST_SWEEP(F, K, P, NF, NOF, ISO, LIST)                              \
   template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO, LIST>(        \
       const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *, \
-      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *);
+      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *, uint8_t *);
 #define PF_INST_SWEEP_ALL(F, K, P, ISO)          \
   PF_INST_SWEEP(F, K, P, 4, false, ISO, false)   \
   PF_INST_SWEEP(F, K, P, 6, false, ISO, false)   \

@@ -575,3 +575,28 @@ def test_diffusion_list_length(tmp_path, form, bc):
     assert 0 < want < nx * ny * nz
     assert listed == want, f"listed {listed}, host count {want}"
     assert again == listed
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+def test_cutoff_change_between_split_sweeps(tmp_path, form):
+    """The split sweep's classification reads, from the previous sweep's per-cell byte, how many own
+    slots are above the cutoff; a new cutoff between steps must make it count them again."""
+    shape = (64, 20, 12)
+    banner(f"cutoff change between split sweeps form={form}")
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=5, nuclei=4)
+
+    def gpu(g, q):
+        g.step(2)
+        q.cutoff = 0.02
+        g.set_params(q)
+        g.step(2)
+
+    def orc(o, q):
+        o.step(2)
+        q.cutoff = 0.02
+        o.set_params(q)
+        o.step(2)
+
+    gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p, gpu, orc)
+    assert_same(gs, os_, 4, "cutoff change")
