@@ -68,6 +68,7 @@ struct pf_handle {
   uint8_t *unit = nullptr;   // scratch of the classification: per-cell unit-vector phase
   unsigned *bulk_count = nullptr;  // [4] bulk cells per phase of the last classification
   bool flag_valid = false;   // flags describe the current value_phase and stencil
+  bool unit_valid = false;   // unit bytes describe the current value_phase and bulk_mask
   int flag_nf = 0;           // stencil faces the flags were computed for
   int diff_by = -1;          // rows of the bulk diffusion tile (0: storage order, -1: by size)
   bool tune_log = false;     // print the plane-marching shape timings (PFAMD_TUNE_LOG=1)
@@ -83,7 +84,7 @@ struct pf_handle {
   int *glist = nullptr;      // indices of the listed cells (sweep: row tiles, diffusion: storage)
   int *gblock = nullptr;     // per-list-block offsets into glist; [nblocks] = list length
   bool sweep_split = true;   // copy cells classified and written apart from the listed sweep
-  uint8_t *sflag = nullptr;  // per cell: 1 = copied by the last sweep
+  uint8_t *sflag = nullptr;  // per cell: bit 0 copied by the last sweep, bits 1-5 slot_summary
   int sweep_rows = -1;       // row tiles of the sweep's list (0: storage order, -1: by size)
   bool sweep_queue = true;   // listed sweep: per-XCD work counters instead of a fixed stride
   int *squeue = nullptr;     // those counters, 128 B apart
@@ -188,6 +189,7 @@ void update_bulk_mask(pf_handle *h) {
     }
     if (ok) m |= 1u << (p + 1);
   }
+  if (m != h->bulk_mask) h->unit_valid = false;  // the unit bytes test the mask's phases
   h->bulk_mask = m;
   h->flag_valid = false;
 }
@@ -319,15 +321,19 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true>), dim3(ng), dim3(kSweepBS),
                        0, h->stream, h->id[a], phi1, h->id[b], phi2, static_cast<T *>(h->pfrac),
                        c0, cs, h->d.nx, h->d.ny, h->d.nz, h->N, h->glist, h->gblock + nl,
-                       h->sweep_queue ? h->squeue : static_cast<int *>(nullptr), h->sflag);
+                       h->sweep_queue ? h->squeue : static_cast<int *>(nullptr), h->sflag,
+                       h->unit, h->bulk_mask);
     h->sweep_hist = true;
   } else {
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, ISO, false>),
                        dim3(blocks_for(h->N, kSweepBS)), dim3(kSweepBS), 0, h->stream, h->id[a],
                        phi1, h->id[b], phi2, static_cast<T *>(h->pfrac), c0, cs, h->d.nx, h->d.ny,
-                       h->d.nz, h->N, nullptr, nullptr, nullptr, nullptr);
+                       h->d.nz, h->N, nullptr, nullptr, nullptr, nullptr, nullptr, 0u);
     h->sweep_hist = false;
+    h->unit_valid = false;  // value_phase rewritten without its unit bytes
   }
+  // (the split sweep keeps the unit bytes current: copy cells keep their value_phase, the listed
+  // sweep writes the bytes of the cells it updates)
   h->cur = b;
   h->flag_valid = false;
   return hipGetLastError();
@@ -500,8 +506,10 @@ hipError_t launch_diffuse_io(pf_handle *h, const DiffConsts<typename F::A> &dc,
     if (!h->flag_valid || h->flag_nf != NF) {
       const hipError_t ec =
           launch_classify<T, P, NF, NOF>(h->stream, pf, h->unit, h->flag, h->d.nx, h->d.ny,
-                                         h->d.nz, h->N, h->bulk_mask, h->bulk_count);
+                                         h->d.nz, h->N, h->bulk_mask, h->bulk_count,
+                                         h->unit_valid);
       if (ec != hipSuccess) return ec;
+      h->unit_valid = true;
       if (h->diff_split) {
         // the plane-marching kernel leaves every cell whose flag is not its bulk phase; the other
         // bulk kernels leave the flag-0 cells
@@ -971,6 +979,7 @@ int pf_upload_state(pf_handle *h, const pf_host_state *s) {
   h->csub[0] = h->csub[1] = 0;
   h->have_state = true;
   h->flag_valid = false;
+  h->unit_valid = false;
   h->sweep_hist = false;
   tables_state_set(h);
   return PF_OK;
@@ -1328,6 +1337,7 @@ int pf_checkpoint_restore(pf_handle *h, const void *src) {
   for (int ph = 0; ph < 4; ++ph) h->code_to_id[ph] = tables[ph];
   h->have_state = true;
   h->flag_valid = false;
+  h->unit_valid = false;
   h->sweep_hist = false;
   tables_state_set(h);
   return PF_OK;
@@ -1394,6 +1404,7 @@ int pf_setup_construct(pf_handle *h, const double *seeds, int G, double reductio
   for (int g = 1; g <= G; ++g) h->code_to_id[2].push_back(200000 + g);
   h->have_state = true;
   h->flag_valid = false;
+  h->unit_valid = false;
   h->sweep_hist = false;
   tables_state_set(h);
   return PF_OK;
@@ -1503,6 +1514,7 @@ int pf_setup_nucleate(pf_handle *h, int n_nuclei, float radius, const double *nu
   if (e != hipSuccess) return fail(h, PF_ERR_HIP, std::string("nucleate: ") + hipGetErrorString(e));
   h->code_to_id[1] = table;
   h->flag_valid = false;
+  h->unit_valid = false;
   h->sweep_hist = false;
   tables_state_set(h);
   return PF_OK;
@@ -1573,6 +1585,7 @@ int pf_copy_state(pf_handle *dst, const pf_handle *src) {
   for (int ph = 0; ph < 4; ++ph) dst->code_to_id[ph] = src->code_to_id[ph];
   dst->have_state = true;
   dst->flag_valid = false;
+  dst->unit_valid = false;
   dst->sweep_hist = false;
   tables_state_set(dst);
   return PF_OK;
@@ -1668,6 +1681,7 @@ int pf_gpu_diffusion_solver(const double *hdiffusivity, const double *hpartition
   h->csub[0] = 0;
   h->have_state = true;
   h->flag_valid = false;
+  h->unit_valid = false;
   tables_state_set(h);  // the reference partitions the uploaded state first (gpu_solver.cpp:210)
   h->NF = 6;  // Diff3D always sums six faces
   rc = diffuse_any(h, 0, num_loop, dt_dxdx);
@@ -1743,6 +1757,7 @@ int handle_diffuse_io(pf_handle *h, const double *cin, double *cout) {
 }
 void handle_state_changed(pf_handle *h) {
   h->flag_valid = false;
+  h->unit_valid = false;
   h->sweep_hist = false;
   tables_state_set(h);
 }

@@ -262,6 +262,10 @@ __global__ __launch_bounds__(256) void unit_phase4_kernel(const T *__restrict__ 
 
 constexpr int kClassifyBlocks = 4096;  // grid-stride, so the per-phase counts take few atomics
 
+__global__ __launch_bounds__(64) void zero_counts_kernel(unsigned *__restrict__ count) {
+  if (threadIdx.x < 4) count[threadIdx.x] = 0u;
+}
+
 // A workgroup's per-phase bulk counts (wave-uniform n1..n3 in each of its four wavefronts) added to
 // count[1..3] with one atomic per phase: atomics on one address serialise, and one per wavefront
 // (3 x 16384 at 512^3) cost as much as the pass itself.  Every thread of the workgroup calls it.
@@ -379,9 +383,12 @@ __global__ __launch_bounds__(256) void classify4_kernel(const uint8_t *__restric
 
 template <class T, int P, int NF, bool NOFLUX>
 hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
-                           int ny, int nz, long ncell, unsigned mask, unsigned *count) {
+                           int ny, int nz, long ncell, unsigned mask, unsigned *count,
+                           bool unit_ready) {
   const unsigned nb = unsigned((ncell + 255) / 256);
-  if (nx % 4 == 0)
+  if (unit_ready)  // the bytes are current: only the counters to clear
+    hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, st, count);
+  else if (nx % 4 == 0)
     hipLaunchKernelGGL((unit_phase4_kernel<T, P>), dim3((nb + 3) / 4), dim3(256), 0, st, pfrac,
                        ncell, mask, unit, count);
   else
@@ -1158,12 +1165,12 @@ __global__ __launch_bounds__(256) void diffuse_retable_kernel(
   StencilLoad<F, P, NF> s;
   stencil_load<F, P, NF>(cin, pfrac, i, ncell, nbi, s);
   CellPF<A> me;
-  partition_cell<F, P>(s.me, copied[i] ? dc_prev : dc, me);
+  partition_cell<F, P>(s.me, (copied[i] & 1u) ? dc_prev : dc, me);
   DelAcc<F, P> acc;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     CellPF<A> nb;
-    partition_cell<F, P>(s.nb[f], copied[nbi[f]] ? dc_prev : dc, nb);
+    partition_cell<F, P>(s.nb[f], (copied[nbi[f]] & 1u) ? dc_prev : dc, nb);
 #pragma unroll
     for (int p = 0; p < P; ++p)
       acc.add(p, face_weight<A>(nb.ph[p] * me.ph[p]) * (nb.c[p] - me.c[p]), dc);
@@ -1364,11 +1371,11 @@ __global__ __launch_bounds__(256) void general_scatter_kernel(const uint8_t *__r
 // explicit instantiations used by the host dispatch (pf_api.hip)
 #define PF_INST_CLASSIFY(T, P)                                                                 \
   template hipError_t launch_classify<T, P, 4, false>(hipStream_t, const T *, uint8_t *, uint8_t *, \
-                                                      int, int, int, long, unsigned, unsigned *);  \
+                                                      int, int, int, long, unsigned, unsigned *, bool);  \
   template hipError_t launch_classify<T, P, 6, false>(hipStream_t, const T *, uint8_t *, uint8_t *, \
-                                                      int, int, int, long, unsigned, unsigned *);  \
+                                                      int, int, int, long, unsigned, unsigned *, bool);  \
   template hipError_t launch_classify<T, P, 6, true>(hipStream_t, const T *, uint8_t *, uint8_t *,  \
-                                                     int, int, int, long, unsigned, unsigned *);
+                                                     int, int, int, long, unsigned, unsigned *, bool);
 PF_INST_CLASSIFY(double, 2)
 PF_INST_CLASSIFY(double, 3)
 PF_INST_CLASSIFY(float, 2)

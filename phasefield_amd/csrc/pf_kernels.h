@@ -17,7 +17,8 @@ __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const PFConsts<typename F::A> cs, const int nx, const int ny,
                                 const int nz, const long ncell, const int *__restrict__ list,
                                 const int *__restrict__ count, int *__restrict__ queue,
-                                uint8_t *__restrict__ sflag);
+                                uint8_t *__restrict__ sflag, uint8_t *__restrict__ unit,
+                                const unsigned umask);
 // the LIST walk's per-XCD counters (queue[32 * xcd], xcd < 8) set to zero: one workgroup of 8
 __global__ void zero_queue_kernel(int *__restrict__ queue);
 
@@ -95,7 +96,7 @@ __global__ void diffuse_general_kernel(const typename F::C *__restrict__ cin,
 
 // The first sub-step after a sweep whose partition tables differ from those the stored
 // concentrations were last partitioned with: the full formula for every cell, each cell's partition
-// taken with the previous tables (dc_prev) if the sweep copied it (copied[cell] == 1), else with the
+// taken with the previous tables (dc_prev) if the sweep copied it (bit 0 of copied[cell]), else with the
 // current ones.  Diffusivities, factor and rounding key come from dc.
 template <class F, int P, int NF, bool NOFLUX>
 __global__ void diffuse_retable_kernel(const typename F::C *__restrict__ cin,
@@ -181,10 +182,12 @@ __global__ void general_scatter_kernel(const uint8_t *__restrict__ flag,
                                        const ListOrder order);
 
 // Bulk flags of a value_phase field into `flag` and the bulk cells per phase into count[1..3]
-// (two kernels; `unit` is an N-byte scratch array)
+// (two kernels; `unit` is an N-byte array of unit-phase bytes, computed here unless `unit_ready`
+// says it already describes this value_phase and mask)
 template <class T, int P, int NF, bool NOFLUX>
 hipError_t launch_classify(hipStream_t st, const T *pfrac, uint8_t *unit, uint8_t *flag, int nx,
-                           int ny, int nz, long ncell, unsigned mask, unsigned *count);
+                           int ny, int nz, long ncell, unsigned mask, unsigned *count,
+                           bool unit_ready = false);
 
 template <class F, int P, int NF, bool NOFLUX, int BY, bool SPLIT>
 __global__ void diffuse_bulk_kernel(const typename F::C *__restrict__ cin,

@@ -47,7 +47,8 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     typename F::T *__restrict__ pfrac, const typename F::C *__restrict__ conc0,
     const PFConsts<typename F::A> cs, const int nx, const int ny, const int nz,
     const long ncell, const int *__restrict__ list, const int *__restrict__ count,
-    int *__restrict__ queue, uint8_t *__restrict__ sflag) {
+    int *__restrict__ queue, uint8_t *__restrict__ sflag, uint8_t *__restrict__ unit,
+    const unsigned umask) {
   using T = typename F::T;
   using A = typename F::A;
   __shared__ uint16_t sLg[kStencilMax][kSweepBS];
@@ -71,9 +72,24 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
         slot = s;
       }
     }
+    int up = 0, ones = 0, zeros = 0;
 #pragma unroll
-    for (int p = 0; p < P; ++p) pfrac[cell * P + p] = T(vp[p + 1] * inv);
-    if (LIST) sflag[cell] = slot_summary(above, slot);
+    for (int p = 0; p < P; ++p) {
+      const T v = T(vp[p + 1] * inv);
+      pfrac[cell * P + p] = v;
+      if (v == T(1)) {
+        ++ones;
+        up = p + 1;
+      } else if (v == T(0)) {
+        ++zeros;
+      }
+    }
+    if (LIST) {
+      sflag[cell] = slot_summary(above, slot);
+      // the diffusion flag pass's unit-phase byte of the new value_phase (unit_phase_kernel's
+      // rule), so that pass need not read value_phase again after a split sweep
+      unit[cell] = uint8_t(ones == 1 && zeros == P - 1 && ((umask >> up) & 1u) ? up : 0);
+    }
   };
   // a cell whose union has one entry keeps its row (solver.cpp:207-216)
   auto copy_cell = [&](const long cell, const T (&v0)[K]) {
@@ -608,7 +624,8 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
 #define PF_INST_SWEEP(F, K, P, NF, NOF, ISO, LIST)                              \
   template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO, LIST>(        \
       const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *, \
-      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *, uint8_t *);
+      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *, uint8_t *, \
+      uint8_t *, unsigned);
 #define PF_INST_SWEEP_ALL(F, K, P, ISO)          \
   PF_INST_SWEEP(F, K, P, 4, false, ISO, false)   \
   PF_INST_SWEEP(F, K, P, 6, false, ISO, false)   \

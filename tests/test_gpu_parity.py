@@ -600,3 +600,27 @@ def test_cutoff_change_between_split_sweeps(tmp_path, form):
 
     gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p, gpu, orc)
     assert_same(gs, os_, 4, "cutoff change")
+
+
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+def test_bulk_mask_change_between_steps(tmp_path, form):
+    """The split sweep keeps the diffusion flag pass's unit-phase bytes current from step to step; a
+    partition table that stops being the identity for a phase (k(2,2) = 1.1) changes the bulk mask,
+    which must recompute them, and so must the change back."""
+    shape = (64, 20, 12)
+    banner(f"bulk mask change between steps form={form}")
+    nx, ny, nz = shape
+    m, p, st = _setup_state(tmp_path, nx=nx, ny=ny, nz=nz, grains=5, nuclei=4)
+
+    def run(d, q, setp):
+        d.step(2)
+        q.part_k[0][1][1] = 1.1
+        setp(q)
+        d.step(2)
+        q.part_k[0][1][1] = 1.0
+        setp(q)
+        d.step(2)
+
+    gs, os_ = run_pair(shape, 4, 2, 1, 0, form, st, p, lambda g, q: run(g, q, g.set_params),
+                       lambda o, q: run(o, q, o.set_params))
+    assert_same(gs, os_, 4, "bulk mask change")
