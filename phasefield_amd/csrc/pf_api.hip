@@ -88,6 +88,7 @@ struct pf_handle {
   int sweep_rows = -1;       // row tiles of the sweep's list (0: storage order, -1: by size)
   bool sweep_queue = true;   // listed sweep: per-XCD work counters instead of a fixed stride
   int *squeue = nullptr;     // those counters, 128 B apart
+  int *olist = nullptr;      // K = 4 split sweep: cells for the general path; [N]: their count
   int num_cu = 0;            // compute units of the handle's device
   bool sweep_hist = false;   // sflag describes the last sweep of the current buffers
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
@@ -305,11 +306,19 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     // entries in process stay one compact front (512^3: 190 -> 83 B read per listed cell, mixed).
     // With a fixed stride the workgroups drift apart as their cells' work differs, and the front
     // outgrows the XCD's L2.
+    // FP64, K = 4: the register path of unions up to four entries is a kernel of its own (no LDS
+    // arrays, 81 instead of 106 VGPRs, 6 instead of 486 spilled SGPRs), and the few cells whose
+    // union outgrows it are appended to an overflow list that the general path then takes (512^3:
+    // 2128 -> 1944 + 51 us).  The mixed form's listed sweep is VALU-bound at 93 % busy, so the
+    // extra occupancy buys nothing there and the second launch costs 48 us: it keeps both paths in
+    // one kernel.
+    constexpr int MODE = (K == 4 && !F::kMixed) ? 1 : 0;
     static int per_cu = 0;  // one value per instantiation
     if (per_cu == 0) {
       int n = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-              &n, pf_sweep_kernel<F, K, P, NF, NOF, false, true>, kSweepBS, 0) != hipSuccess ||
+              &n, pf_sweep_kernel<F, K, P, NF, NOF, false, true, MODE>, kSweepBS, 0) !=
+              hipSuccess ||
           n < 1)
         n = 1;
       per_cu = n;
@@ -318,17 +327,27 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
                                                 long(std::max(h->num_cu, 1)) * per_cu));
     if (h->sweep_queue)
       hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(8), 0, h->stream, h->squeue);
-    hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true>), dim3(ng), dim3(kSweepBS),
-                       0, h->stream, h->id[a], phi1, h->id[b], phi2, static_cast<T *>(h->pfrac),
-                       c0, cs, h->d.nx, h->d.ny, h->d.nz, h->N, h->glist, h->gblock + nl,
-                       h->sweep_queue ? h->squeue : static_cast<int *>(nullptr), h->sflag,
-                       h->unit, h->bulk_mask);
+    if (MODE == 1)  // the overflow count, olist[N]
+      hipLaunchKernelGGL(zero_counts_kernel, dim3(1), dim3(64), 0, h->stream,
+                         reinterpret_cast<unsigned *>(h->olist + h->N));
+    hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true, MODE>), dim3(ng),
+                       dim3(kSweepBS), 0, h->stream, h->id[a], phi1, h->id[b], phi2,
+                       static_cast<T *>(h->pfrac), c0, cs, h->d.nx, h->d.ny, h->d.nz, h->N, h->glist,
+                       h->gblock + nl, h->sweep_queue ? h->squeue : static_cast<int *>(nullptr),
+                       h->sflag, h->unit, h->bulk_mask, h->olist);
+    if (MODE == 1)
+      hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, false, true, 2>),
+                         dim3(unsigned(std::max(h->num_cu, 1))), dim3(kSweepBS), 0, h->stream,
+                         h->id[a], phi1, h->id[b], phi2, static_cast<T *>(h->pfrac), c0, cs,
+                         h->d.nx, h->d.ny, h->d.nz, h->N, h->olist, h->olist + h->N,
+                         static_cast<int *>(nullptr), h->sflag, h->unit, h->bulk_mask,
+                         static_cast<int *>(nullptr));
     h->sweep_hist = true;
   } else {
     hipLaunchKernelGGL((pf_sweep_kernel<F, K, P, NF, NOF, ISO, false>),
                        dim3(blocks_for(h->N, kSweepBS)), dim3(kSweepBS), 0, h->stream, h->id[a],
                        phi1, h->id[b], phi2, static_cast<T *>(h->pfrac), c0, cs, h->d.nx, h->d.ny,
-                       h->d.nz, h->N, nullptr, nullptr, nullptr, nullptr, nullptr, 0u);
+                       h->d.nz, h->N, nullptr, nullptr, nullptr, nullptr, nullptr, 0u, nullptr);
     h->sweep_hist = false;
     h->unit_valid = false;  // value_phase rewritten without its unit bytes
   }
@@ -684,6 +703,7 @@ void free_all(pf_handle *h) {
   if (h->ccount) hipFree(h->ccount);
   if (h->sflag) hipFree(h->sflag);
   if (h->squeue) hipFree(h->squeue);
+  if (h->olist) hipFree(h->olist);
   if (h->gblock) hipFree(h->gblock);
   if (h->red) hipFree(h->red);
   for (int k = 0; k < 3; ++k)
@@ -795,6 +815,10 @@ int pf_create(const pf_desc *desc, pf_handle **out) {
     return rc;
   if ((rc = chk(hipMalloc(&h->sflag, N), "alloc sweep copy flags")) != PF_OK) return rc;
   if ((rc = chk(hipMalloc(&h->squeue, 8 * 32 * sizeof(int)), "alloc sweep counters")) != PF_OK)
+    return rc;
+  if (d.depth == 4 && !is_mixed(h.get()) &&
+      (rc = chk(hipMalloc(&h->olist, (h->N + 4) * sizeof(int)), "alloc sweep overflow list")) !=
+          PF_OK)
     return rc;
   if ((rc = chk(hipMalloc(&h->gblock, (list_blocks(h->N) + 1) * sizeof(int)),
                 "alloc list offsets")) != PF_OK)

@@ -8,7 +8,10 @@ namespace pfamd {
 constexpr int kDiffTY = 8;     // rows of the diffusion tile (64 x kDiffTY threads)
 constexpr int kSweepBS = 128;  // threads per phase-field sweep block (one cell each)
 
-template <class F, int K, int P, int NF, bool NOFLUX, bool ISO, bool LIST>
+// MODE (LIST, K = 4, FP64 form): 0 both paths of the union; 1 the register path only, a cell whose union
+// outgrows it appended to olist (its count in olist[ncell], zeroed before); 2 the general path
+// only, over list = olist, count = olist + ncell
+template <class F, int K, int P, int NF, bool NOFLUX, bool ISO, bool LIST, int MODE = 0>
 __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const typename F::T *__restrict__ phi1,
                                 uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
@@ -18,7 +21,9 @@ __global__ void pf_sweep_kernel(const uint16_t *__restrict__ id1,
                                 const int nz, const long ncell, const int *__restrict__ list,
                                 const int *__restrict__ count, int *__restrict__ queue,
                                 uint8_t *__restrict__ sflag, uint8_t *__restrict__ unit,
-                                const unsigned umask);
+                                const unsigned umask, int *__restrict__ olist);
+// count[0..3] set to zero (one workgroup of 64)
+__global__ void zero_counts_kernel(unsigned *__restrict__ count);
 // the LIST walk's per-XCD counters (queue[32 * xcd], xcd < 8) set to zero: one workgroup of 8
 __global__ void zero_queue_kernel(int *__restrict__ queue);
 

@@ -40,7 +40,7 @@ __device__ inline A pair_pick(const A (&tab)[4][4], int pa, int pb) {
 }
 
 // LIST: the cells come from list[0 .. *count) (grid-stride); otherwise one cell per thread.
-template <class F, int K, int P, int NF, bool NOFLUX, bool ISO, bool LIST>
+template <class F, int K, int P, int NF, bool NOFLUX, bool ISO, bool LIST, int MODE>
 __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     const uint16_t *__restrict__ id1, const typename F::T *__restrict__ phi1,
     uint16_t *__restrict__ id2, typename F::T *__restrict__ phi2,
@@ -48,7 +48,8 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     const PFConsts<typename F::A> cs, const int nx, const int ny, const int nz,
     const long ncell, const int *__restrict__ list, const int *__restrict__ count,
     int *__restrict__ queue, uint8_t *__restrict__ sflag, uint8_t *__restrict__ unit,
-    const unsigned umask) {
+    const unsigned umask, int *__restrict__ olist) {
+  static_assert(MODE == 0 || (LIST && K == 4 && !ISO), "the split walk is for K = 4 lists");
   using T = typename F::T;
   using A = typename F::A;
   __shared__ uint16_t sLg[kStencilMax][kSweepBS];
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
     return acc;
   };
 
-  if constexpr (K == 4 && !ISO) {
+  if constexpr (K == 4 && !ISO && MODE != 2) {
     // Registers for a union of at most four entries (almost every listed cell: two grains at a
     // grain boundary, three at a triple line or an austenite front), the same algorithm as the
     // general path below: own slots at their slot positions, first-match search, appends at
@@ -360,6 +361,17 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
       store_cell(cell, go, out, vp, sum);
       return;
     }
+    if constexpr (MODE == 1) {
+      // The general path is a kernel of its own (MODE 2), so this one keeps no LDS arrays and
+      // fewer registers: the cell goes to the overflow list, one atomic per wavefront.
+      const unsigned long long act = __ballot(1);
+      const int lane = t & 63, leader = __ffsll(static_cast<long long>(act)) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(olist + ncell, __popcll(act));
+      base = __shfl(base, leader);
+      olist[base + __popcll(act & ((1ull << lane) - 1ull))] = int(cell);
+      return;
+    }
   }
 
   for (int e = 0; e < kStencilMax; ++e) {
@@ -457,7 +469,10 @@ __global__ __launch_bounds__(kSweepBS) void pf_sweep_kernel(
   }
   store_cell(cell, go, out, vp, sum);
   };
-  if (LIST) {
+  if (MODE == 2) {  // the overflow list: grid-stride
+    const int n = *count;
+    for (int k = blockIdx.x * kSweepBS + t; k < n; k += gridDim.x * kSweepBS) body(list[k]);
+  } else if (LIST) {
     // Workgroup b runs on XCD b % 8: each XCD walks a contiguous eighth of the list (row-tile
     // order, all its workgroups resident, one front of consecutive entries), so the neighbour
     // rows of a workgroup's cells are in its own XCD's L2.  (Fewer than 8 workgroups: one plain
@@ -621,11 +636,12 @@ __global__ __launch_bounds__(256) void sweep_classify_kernel(
   }
 }
 
-#define PF_INST_SWEEP(F, K, P, NF, NOF, ISO, LIST)                              \
-  template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO, LIST>(        \
-      const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *, \
-      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *, uint8_t *, \
-      uint8_t *, unsigned);
+#define PF_INST_SWEEP_M(F, K, P, NF, NOF, ISO, LIST, MODE)                                       \
+  template __global__ void pf_sweep_kernel<F, K, P, NF, NOF, ISO, LIST, MODE>(                   \
+      const uint16_t *, const F::T *, uint16_t *, F::T *, F::T *, const F::C *,                 \
+      const PFConsts<F::A>, int, int, int, long, const int *, const int *, int *, uint8_t *,    \
+      uint8_t *, unsigned, int *);
+#define PF_INST_SWEEP(F, K, P, NF, NOF, ISO, LIST) PF_INST_SWEEP_M(F, K, P, NF, NOF, ISO, LIST, 0)
 #define PF_INST_SWEEP_ALL(F, K, P, ISO)          \
   PF_INST_SWEEP(F, K, P, 4, false, ISO, false)   \
   PF_INST_SWEEP(F, K, P, 6, false, ISO, false)   \

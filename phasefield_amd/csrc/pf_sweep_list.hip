@@ -11,6 +11,16 @@ PF_INST_SWEEP_LIST(FormMixed, 4, 2)
 PF_INST_SWEEP_LIST(FormMixed, 4, 3)
 PF_INST_SWEEP_LIST(FormMixed, 6, 2)
 PF_INST_SWEEP_LIST(FormMixed, 6, 3)
+// FP64, K = 4 lists: the register path with its overflow list, then the general path over it
+#define PF_INST_SWEEP_SPLIT(F, P)                            \
+  PF_INST_SWEEP_M(F, 4, P, 4, false, false, true, 1)         \
+  PF_INST_SWEEP_M(F, 4, P, 6, false, false, true, 1)         \
+  PF_INST_SWEEP_M(F, 4, P, 6, true, false, true, 1)          \
+  PF_INST_SWEEP_M(F, 4, P, 4, false, false, true, 2)         \
+  PF_INST_SWEEP_M(F, 4, P, 6, false, false, true, 2)         \
+  PF_INST_SWEEP_M(F, 4, P, 6, true, false, true, 2)
+PF_INST_SWEEP_SPLIT(FormF64, 2)
+PF_INST_SWEEP_SPLIT(FormF64, 3)
 PF_INST_SWEEP_CLASSIFY(FormF64, 4)
 PF_INST_SWEEP_CLASSIFY(FormF64, 6)
 PF_INST_SWEEP_CLASSIFY(FormMixed, 4)
