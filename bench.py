@@ -452,7 +452,7 @@ def main() -> int:
         print(json.dumps({"calibration_copy_bytes": n * 8}))
         return 0
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "phasefield_amd runs one domain on one GPU; --gpus must be 1 "
                                    "(multi-GPU halo exchange is not built)"}))
         return 2
     # the oracle's OpenMP runtime reads these when it starts (cpu_baseline)

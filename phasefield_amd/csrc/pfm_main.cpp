@@ -94,7 +94,7 @@ int main(int argc, char **argv) {
   pf_model_desc(m, &d);
   int64_t gpu_flag = 1, G = 0, n_iso = 0, steps = -1, seed_nuc = 0, seed_mn = 0, nsol = 1;
   pf_model_get_int(m, "gpu_flag", &gpu_flag);
-  if (!gpu_flag) std::printf("GPU_Flag is 0 in the input; this build always runs on the GPU\n");
+  if (!gpu_flag) std::printf("GPU_Flag is 0 in the input; pfm always runs on the GPU\n");
   pf_model_get_int(m, "num_grain1", &G);
   pf_model_get_int(m, "num_init_iterations", &n_iso);
   pf_model_get_int(m, "steps", &steps);
