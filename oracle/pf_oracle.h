@@ -64,6 +64,12 @@ void orc_dual_phase_fraction(void *h, double *value, double *deriv4);
 void orc_dual_conc(void *h, double *c, double *dc /* [4][N] */);
 
 /* ---- setup restatements ---------------------------------------------------------------------- */
+/* Parity unpinned at the Blitz++ boundary: the reference draws its seed coordinates and its
+ * nucleation noise from Blitz ranlib streams and picks nuclei with Blitz's 3D maxIndex, none of
+ * which is available here or fixed by a reference test.  These restatements use std::mt19937 seeds
+ * (pinned by the C++ standard's 10000th output), a counter-based noise and the first maximum in
+ * storage order, each pinned by brute force; the device setup is compared against them, and the
+ * step's parity does not depend on them (GPU and oracle start from the same state). */
 /* Tessellation (MicrostructureConstruction.cpp:68-109) over 1-based cells; grain_index [N]. */
 void orc_tessellation(int nx, int ny, int nz, const double *seeds, int n_grains, double reduction,
                       int32_t *grain_index);
