@@ -90,6 +90,7 @@ struct pf_handle {
   int *squeue = nullptr;     // those counters, 128 B apart
   int *olist = nullptr;      // K = 4 split sweep: cells for the general path; [N]: their count
   int num_cu = 0;            // compute units of the handle's device
+  std::map<const void *, int> occupancy;  // resident workgroups per CU, by kernel, on that device
   bool sweep_hist = false;   // sflag describes the last sweep of the current buffers
   unsigned bulk_mask = 0;    // bit p: phase p may take the bulk-cell reduction
   // The reference stores the partitioned concentrations (value_conc1) and reads them until they are
@@ -313,7 +314,9 @@ hipError_t launch_sweep(pf_handle *h, const PFConsts<typename F::A> &cs) {
     // extra occupancy buys nothing there and the second launch costs 48 us: it keeps both paths in
     // one kernel.
     constexpr int MODE = (K == 4 && !F::kMixed) ? 1 : 0;
-    static int per_cu = 0;  // one value per instantiation
+    const void *kfn =
+        reinterpret_cast<const void *>(pf_sweep_kernel<F, K, P, NF, NOF, false, true, MODE>);
+    int &per_cu = h->occupancy[kfn];
     if (per_cu == 0) {
       int n = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
