@@ -96,6 +96,20 @@ def test_phase_field_sweep_bitwise(shape, bc, K, P, base_params, rng):
     assert_same(gs, os_, K, "sweep")
 
 
+@pytest.mark.parametrize("shape,bc", SHAPES)
+@pytest.mark.parametrize("K,P", [(4, 2), (6, 2), (4, 3), (6, 3)])
+def test_mixed_steps_random_states_bitwise(shape, bc, K, P, base_params, rng):
+    """The mixed form's full steps (sweep + sub-steps) from random active-set states, against the
+    oracle's mixed form: every depth and phase count, periodic and no-flux z, 2D, partial tiles;
+    up to K grains per cell, so unions beyond four entries take the sweep's general path."""
+    banner(f"mixed steps {shape} bc={bc} K={K} P={P}")
+    params = with_phase3(base_params) if P == 3 else base_params
+    st = helpers.random_state(rng, *shape, K, nphase=P)
+    gs, os_ = run_pair(shape, K, P, 1, bc, oracle.ORC_MIXED, st, params,
+                       lambda g, p: g.step(2), lambda o, p: o.step(2))
+    assert_same(gs, os_, K, "mixed steps")
+
+
 def _setup_state(tmp_path, **over):
     m = helpers.small_model(str(tmp_path), over.pop("workload", "fec_3d_256"), **over)
     p = helpers.model_params(m)
