@@ -110,6 +110,20 @@ def test_mixed_steps_random_states_bitwise(shape, bc, K, P, base_params, rng):
     assert_same(gs, os_, K, "mixed steps")
 
 
+@pytest.mark.parametrize("form", [oracle.ORC_FP64, oracle.ORC_MIXED])
+@pytest.mark.parametrize("shape,bc", [((2, 2, 2), 0), ((2, 2, 2), 1), ((3, 5, 4), 0), ((4, 4, 2), 1),
+                                      ((5, 3, 3), 1), ((2, 6, 1), 0), ((8, 2, 3), 0)])
+def test_tiny_domains_bitwise(shape, bc, form, base_params, rng):
+    """Domains of two to eight cells per axis, where a periodic neighbour wraps onto the cell's
+    other neighbour or (no-flux, nz = 2) the two z faces meet the same cell, and every cell is at
+    a tile and chunk edge: full steps from random states, both forms, bitwise."""
+    banner(f"tiny domain {shape} bc={bc} form={form}")
+    st = helpers.random_state(rng, *shape, 4)
+    gs, os_ = run_pair(shape, 4, 2, 1, bc, form, st, base_params,
+                       lambda g, p: g.step(2), lambda o, p: o.step(2))
+    assert_same(gs, os_, 4, "tiny domain")
+
+
 def _setup_state(tmp_path, **over):
     m = helpers.small_model(str(tmp_path), over.pop("workload", "fec_3d_256"), **over)
     p = helpers.model_params(m)
