@@ -99,6 +99,40 @@ def test_fec_2d_256_1000_steps_bitwise(tmp_path):
     assert gs["phase"].reshape(-1, 3)[:, 0].sum() > s0["phase"].reshape(-1, 3)[:, 0].sum()
 
 
+@pytest.mark.slow
+def test_fec_2d_256_1000_steps_mixed_bitwise(tmp_path):
+    """The same workload and horizon in the mixed form against the oracle's mixed form: 200 000
+    sub-steps of stochastically rounded compensation codes (keys counted per sub-step), list
+    rebuilds after every sweep; identical maps, values, concentrations and codes."""
+    banner("fec_2d_256 mixed: 1000 full steps vs oracle")
+    m = Model(configs.write_inputs(str(tmp_path), "fec_2d_256"))
+    p = m.update_parameter()
+    d = m.desc()
+    d.precision = 1
+    g = Domain.from_desc(d)
+    g.set_params(p)
+    f = Domain.from_desc(m.desc())
+    f.set_params(p)
+    f.construct(m.voronoi_seeds(), m.get_real("reduction"), m.get_int("num_init_iterations"),
+                [m.get_real(f"init_conc{i}") for i in range(3)])
+    f.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"),
+               (m.get_real("nucleus_conc"), -1.0), m.get_int("nucleation_seed"))
+    s0 = f.download()
+    f.close()
+    g.upload(s0["index"], s0["value"], s0["phase"], s0["conc"])
+    o = oracle.OracleDomain(d.nx, d.ny, d.nz, d.depth, 1, oracle.ORC_MIXED)
+    o.set_params(oracle.as_orc_params(p))
+    o.set_state(s0["index"], s0["value"], s0["phase"], s0["conc"])
+    g.step(1000)
+    o.step(1000)
+    gs, os_ = g.download(), o.get_state()
+    assert np.array_equal(gs["index"], os_["index"]), "active-set maps differ"
+    for key in ("value", "phase", "conc", "conc_comp"):
+        assert np.array_equal(gs[key], os_[key]), key
+    assert gs["phase"].reshape(-1, 3)[:, 0].sum() > s0["phase"].reshape(-1, 3)[:, 0].sum()
+    g.close()
+
+
 def _vti_fields(s: dict, d: PfDesc):
     """korn and float32 carbon of a downloaded state, as write_data.cpp:42-82 defines them."""
     idx = s["index"].reshape(-1, d.depth)
