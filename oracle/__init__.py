@@ -50,6 +50,7 @@ def lib():
             "orc_destroy": (None, [C.c_void_p]),
             "orc_set_params": (None, [C.c_void_p, P(OrcParams)]),
             "orc_set_state": (None, [C.c_void_p, P(C.c_int32), P(_D), P(_D), P(_D)]),
+            "orc_set_comp": (None, [C.c_void_p, P(_D)]),
             "orc_get_state": (None, [C.c_void_p, P(C.c_int32), P(_D), P(_D), P(_D), P(_D), P(_D)]),
             "orc_pf_sweep": (None, [C.c_void_p]),
             "orc_diffusion_substeps": (None, [C.c_void_p, _I, _I, _D]),
@@ -123,6 +124,11 @@ class OracleDomain:
         phase = np.ascontiguousarray(phase, np.float64).reshape(-1)
         conc = np.ascontiguousarray(conc, np.float64).reshape(-1)
         lib().orc_set_state(self._h, _ip(index), _dp(value), _dp(phase), _dp(conc))
+
+    def set_comp(self, comp):
+        comp = np.ascontiguousarray(comp, np.float64).reshape(-1)
+        assert comp.size == self.N * self.S
+        lib().orc_set_comp(self._h, _dp(comp))
 
     def get_state(self):
         N, K, S = self.N, self.K, self.S

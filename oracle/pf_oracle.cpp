@@ -18,7 +18,7 @@
 //
 // Three number forms share the code: FP64 (the reference), MIXED (FP32 storage and arithmetic,
 // FP64 for the clamp/prune decision and renormalisation, Kahan-compensated concentration update
-// with the compensation held as an 8-bit fixed-point fraction of an ulp, rounded stochastically)
+// with the compensation held as a 4-bit fixed-point fraction of an ulp, rounded stochastically)
 // and DUAL (FP64 with forward-mode derivatives, the reference gradient for the adjoint).
 
 #include "pf_oracle.h"
@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -159,6 +160,7 @@ struct DomainBase {
   int nsolute = 1;  // solutes the domain carries
   virtual void set_params(const orc_params *p) = 0;
   virtual void set_state(const int32_t *, const double *, const double *, const double *) = 0;
+  virtual void set_comp(const double *) = 0;
   virtual void get_state(int32_t *, double *, double *, double *, double *, double *) = 0;
   virtual void pf_sweep() = 0;
   virtual void diffuse(int s, int n, double factor) = 0;
@@ -285,6 +287,14 @@ template <class F> struct Domain : DomainBase {
       }
     }
     repartition();
+  }
+
+  // the mixed form's stored compensations [S][N] (values on its code grid, as a download of a
+  // mixed device state gives them); the FP64 and dual forms keep none
+  void set_comp(const double *c) override {
+    if (!std::is_same<S, float>::value) return;
+    for (int s = 0; s < NS; ++s)
+      for (long n = 0; n < N; ++n) comp[s][n] = cvt<S>(c[s * N + n]);
   }
 
   void repartition() {
@@ -754,6 +764,7 @@ void orc_set_state(void *h, const int32_t *index, const double *value, const dou
                    const double *conc) {
   dom(h)->set_state(index, value, phase, conc);
 }
+void orc_set_comp(void *h, const double *comp) { dom(h)->set_comp(comp); }
 void orc_get_state(void *h, int32_t *index, double *value, double *phase, double *conc,
                    double *comp, double *phase_conc) {
   dom(h)->get_state(index, value, phase, conc, comp, phase_conc);

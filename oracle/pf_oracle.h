@@ -40,6 +40,9 @@ void orc_set_params(void *h, const orc_params *p);
  * (Partitioning of conc with phase), as after Construct (MicrostructureConstruction.cpp:54-62). */
 void orc_set_state(void *h, const int32_t *index, const double *value, const double *phase,
                    const double *conc);
+/* Mixed form: the stored compensations comp [S][N] after orc_set_state (values on the 4-bit code
+ * grid, as a mixed device download gives them); ignored by the other forms. */
+void orc_set_comp(void *h, const double *comp);
 /* conc: stored total concentration (the mixed form's effective value is conc - comp); comp may
  * be NULL; phase_conc [S][N][3] may be NULL. */
 void orc_get_state(void *h, int32_t *index, double *value, double *phase, double *conc,

@@ -281,3 +281,48 @@ def test_fec_3d_256_late_steps_bitwise(tmp_path):
     o.step(5)
     assert_same(g.download(), o.get_state(), d.depth, "256^3 steps 1996-2000")
     g.close()
+
+def test_fec_3d_256_late_steps_mixed_bitwise(tmp_path):
+    """The same horizon in the mixed form: the device runs 1995 mixed steps from the seeded state;
+    that state (FP32 values and compensation codes) is uploaded into a fresh mixed domain and handed
+    to the oracle's mixed form, whose rounding keys both count from the hand-over, and both run
+    steps 1996-1998; everything, compensations included, agrees bitwise."""
+    banner("fec_3d_256 mixed: steps 1996-1998 vs oracle")
+    m = Model(configs.write_inputs(str(tmp_path), "fec_3d_256"))
+    p = m.update_parameter()
+    d = m.desc()
+    f = Domain.from_desc(d)
+    f.set_params(p)
+    f.construct(m.voronoi_seeds(), m.get_real("reduction"), m.get_int("num_init_iterations"),
+                [m.get_real(f"init_conc{i}") for i in range(3)])
+    f.nucleate(m.get_int("num_nuclei"), m.get_real("nucleus_radius"),
+               (m.get_real("nucleus_conc"), -1.0), m.get_int("nucleation_seed"))
+    s0 = f.download()
+    f.close()
+    d.precision = 1
+    g = Domain.from_desc(d)
+    g.set_params(p)
+    g.upload(s0["index"], s0["value"], s0["phase"], s0["conc"])
+    f0 = g.reduce()["phase_frac"][0]
+    for k in range(1995):
+        g.step(1)
+        if k % 500 == 499:
+            g.synchronize()
+            banner(f"  step {k + 1}")
+    s = g.download()
+    assert g.reduce()["phase_frac"][0] > f0
+    assert np.any(s["conc_comp"] != 0.0)
+    g.close()
+    h = Domain.from_desc(d)
+    h.set_params(p)
+    h.upload(s["index"], s["value"], s["phase"], s["conc"], s["conc_comp"])
+    o = oracle.OracleDomain(d.nx, d.ny, d.nz, d.depth, 1, oracle.ORC_MIXED)
+    o.set_params(oracle.as_orc_params(p))
+    o.set_state(s["index"], s["value"], s["phase"], s["conc"])
+    o.set_comp(s["conc_comp"])
+    h.step(3)
+    o.step(3)
+    hs, os_ = h.download(), o.get_state()
+    assert_same(hs, os_, d.depth, "256^3 mixed steps 1996-1998")
+    assert np.array_equal(hs["conc_comp"], os_["conc_comp"])
+    h.close()
