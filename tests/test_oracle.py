@@ -671,3 +671,41 @@ def test_union_cap_drops_the_eleventh_grain(params3d, own):
     else:
         assert out[0] == G and v[0] == v.max()
     assert abs(v.sum() - 1.0) < 1e-15
+
+
+def test_set_comp_hands_over_the_mixed_compensation(params3d):
+    """orc_set_comp (the hand-over of a mixed device state's stored compensations): the values come
+    back unchanged, the FP64 form ignores them, and the next sub-step takes them as the Kahan update
+    does: with no flux anywhere, the effective value c - comp of every cell is kept exactly (a
+    compensation on the code grid of up to half an ulp is re-stored without rounding noise)."""
+    rng = np.random.default_rng(5)
+    shape = (6, 5, 4)
+    N = 6 * 5 * 4
+    st = helpers.random_state(rng, *shape, 4)
+    # a uniform concentration and a single phase: no flux anywhere, so del = 0 in every cell
+    idx, val, ph, _ = st
+    idx = np.zeros_like(idx)
+    idx.reshape(N, 4)[:, 0] = 200001
+    val = np.zeros_like(val)
+    val.reshape(N, 4)[:, 0] = 1.0
+    ph = np.zeros_like(ph)
+    ph.reshape(N, 3)[:, 1] = 1.0
+    conc = np.full(N, 0.75)
+    quantum = 2.0 ** (0 - 27)  # 0.75 = 0.75 * 2^0
+    comp = quantum * rng.integers(-4, 5, size=N).astype(np.float64)
+    o = oracle.OracleDomain(*shape, 4, 1, oracle.ORC_MIXED)
+    o.set_params(oracle.as_orc_params(params3d))
+    o.set_state(idx, val, ph, conc)
+    o.set_comp(comp)
+    s = o.get_state()
+    assert np.array_equal(s["conc_comp"], comp)
+    assert np.array_equal(s["conc"], conc)
+    # one sub-step with zero flux: y = -comp, t = c + y (|y| < half an ulp of 0.75 rounds back to c),
+    # comp' = (t - c) - y = comp: the stored state is unchanged and the effective value kept
+    o.diffuse(0, 1, oracle.as_orc_params(params3d).dt_d[0])
+    s1 = o.get_state()
+    assert np.array_equal(s1["conc"] - s1["conc_comp"], conc - comp)
+    f = oracle.OracleDomain(*shape, 4, 1, oracle.ORC_FP64)
+    f.set_state(idx, val, ph, conc)
+    f.set_comp(comp)
+    assert not np.any(f.get_state()["conc_comp"])
