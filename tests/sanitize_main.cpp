@@ -166,6 +166,7 @@ static void oracle_runs(const char *input) {
         comp(size_t(N) * c.S), pc(size_t(N) * c.S * 3);
     orc_get_state(h, idx.data(), val.data(), ph.data(), cc.data(), comp.data(), pc.data());
     orc_set_state(h, idx.data(), val.data(), ph.data(), cc.data());
+    orc_set_comp(h, comp.data());  // every form: the mixed one stores them, the others ignore them
     if (c.form == ORC_DUAL) {
       double v, d4[4];
       orc_dual_phase_fraction(h, &v, d4);
